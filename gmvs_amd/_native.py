@@ -1,0 +1,177 @@
+"""ctypes binding of libgmvs_amd.so (the C ABI declared in include/gmvs_amd.h).
+
+The library is built in-tree by ``gmvs_amd._build.build()``; importing this
+module loads it and raises if it is missing -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libgmvs_amd.so")
+
+MAX_NGB = 32
+OK = 0
+ERR_ARG = 1
+ERR_HIP = 2
+ERR_NOSPACE = 3
+ERR_UNSUPPORTED = 4
+
+INIT_RANDOM = 0
+INIT_DEPTH_NORMAL = 1
+INIT_PRIOR = 2
+
+PREC_FP32 = 0
+
+
+class CCamera(ctypes.Structure):
+    _fields_ = [
+        ("K", ctypes.c_float * 9),
+        ("R", ctypes.c_float * 9),
+        ("t", ctypes.c_float * 3),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("depth_min", ctypes.c_float),
+        ("depth_max", ctypes.c_float),
+    ]
+
+
+class CProblem(ctypes.Structure):
+    _fields_ = [
+        ("num_ngb", ctypes.c_int32),
+        ("ref_image_id", ctypes.c_int32),
+        ("src_image_ids", ctypes.c_int32 * MAX_NGB),
+    ]
+
+
+class CParams(ctypes.Structure):
+    _fields_ = [
+        ("max_iterations", ctypes.c_int32),
+        ("patch_size", ctypes.c_int32),
+        ("num_images", ctypes.c_int32),
+        ("max_image_size", ctypes.c_int32),
+        ("radius_increment", ctypes.c_int32),
+        ("sigma_spatial", ctypes.c_float),
+        ("sigma_color", ctypes.c_float),
+        ("top_k", ctypes.c_int32),
+        ("baseline", ctypes.c_float),
+        ("depth_min", ctypes.c_float),
+        ("depth_max", ctypes.c_float),
+        ("disparity_min", ctypes.c_float),
+        ("disparity_max", ctypes.c_float),
+        ("geom_consistency", ctypes.c_bool),
+        ("multi_geometry", ctypes.c_bool),
+        ("planar_prior", ctypes.c_bool),
+        ("superpixel_filter", ctypes.c_bool),
+    ]
+
+
+class CPMArgs(ctypes.Structure):
+    _fields_ = [
+        ("num_images", ctypes.c_int32),
+        ("cameras", ctypes.POINTER(CCamera)),
+        ("images", ctypes.POINTER(ctypes.c_void_p)),
+        ("depths", ctypes.POINTER(ctypes.c_void_p)),
+        ("ref_normal", ctypes.c_void_p),
+        ("prior_planes", ctypes.c_void_p),
+        ("plane_masks", ctypes.c_void_p),
+        ("planes", ctypes.c_void_p),
+        ("costs", ctypes.c_void_p),
+        ("params", CParams),
+        ("init_mode", ctypes.c_int32),
+        ("geom", ctypes.c_int32),
+        ("seed", ctypes.c_uint32),
+        ("rng_view", ctypes.c_uint32),
+        ("precision", ctypes.c_int32),
+        ("skip_filter", ctypes.c_int32),
+        ("out_depth", ctypes.c_void_p),
+        ("out_normal", ctypes.c_void_p),
+        ("out_cost", ctypes.c_void_p),
+    ]
+
+
+# Every symbol include/gmvs_amd.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "gmvs_last_error",
+    "gmvs_version",
+    "gmvs_load_camera",
+    "gmvs_load_problem",
+    "gmvs_pm_workspace_bytes",
+    "gmvs_pm_run",
+    "gmvs_cost_workspace_bytes",
+    "gmvs_cost_batch",
+    "gmvs_fuse_workspace_bytes",
+    "gmvs_fuse_view",
+    "gmvs_prior_build",
+    "gmvs_delaunay",
+    "gmvs_philox4x32_10",
+)
+
+
+class GmvsError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed (status {code}): {msg}")
+        self.code = code
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it first with `python -c \"import __graft_entry__ as g; "
+            f"g.build()\"` (or python gmvs_amd/_build.py). gmvs_amd has no CPU fallback."
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i32, u32, f32, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_float, ctypes.c_size_t
+    lib.gmvs_last_error.restype = ctypes.c_char_p
+    lib.gmvs_version.restype = ctypes.c_char_p
+    lib.gmvs_load_camera.argtypes = [vp, vp, vp, i32, i32, f32, f32, ctypes.POINTER(CCamera)]
+    lib.gmvs_load_problem.argtypes = [i32, vp, i32, ctypes.POINTER(CProblem)]
+    lib.gmvs_pm_workspace_bytes.argtypes = [i32, i32, i32, i32]
+    lib.gmvs_pm_workspace_bytes.restype = sz
+    lib.gmvs_pm_run.argtypes = [ctypes.POINTER(CPMArgs), vp, sz, vp]
+    lib.gmvs_cost_workspace_bytes.argtypes = [i32, i32, i32, i32]
+    lib.gmvs_cost_workspace_bytes.restype = sz
+    lib.gmvs_cost_batch.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
+                                    ctypes.POINTER(CParams), i32, i32, vp, vp, vp, vp, sz, vp]
+    lib.gmvs_fuse_workspace_bytes.argtypes = [i32, i32]
+    lib.gmvs_fuse_workspace_bytes.restype = sz
+    lib.gmvs_fuse_view.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
+                                   ctypes.POINTER(vp), vp, i32, i32, vp, i32, f32, f32, f32, vp,
+                                   vp, vp, sz, vp]
+    lib.gmvs_prior_build.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,
+                                     ctypes.POINTER(CParams), vp, vp, ctypes.POINTER(i32)]
+    lib.gmvs_delaunay.argtypes = [vp, i32, vp, i32, ctypes.POINTER(i32)]
+    lib.gmvs_philox4x32_10.argtypes = [vp, vp, vp]
+    lib.gmvs_philox4x32_10.restype = None
+    for name in EXPORTS:
+        getattr(lib, name)
+    return lib
+
+
+lib = _load()
+
+
+def check(fn: str, code: int) -> None:
+    if code != OK:
+        msg = lib.gmvs_last_error().decode(errors="replace")
+        if code == ERR_ARG:
+            raise ValueError(f"{fn}: {msg}")
+        raise GmvsError(fn, code, msg)
+
+
+def ptr_array(ptrs: Sequence[Optional[int]]):
+    arr = (ctypes.c_void_p * len(ptrs))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p if p else None
+    return arr
+
+
+def camera_array(cams: Sequence[CCamera]):
+    arr = (CCamera * len(cams))()
+    for i, c in enumerate(cams):
+        arr[i] = c
+    return arr

@@ -1,0 +1,294 @@
+// Multi-view geometric-consistency fusion of depth/normal maps (gfx950).
+//
+// Replaces batch_fusion_kernel + Fuser::run_fusion (gmvs/src/fusion.cu:61-182,
+// 211-275). The arithmetic is the reference's FP32 with every product and
+// sum rounded as written (this file is compiled with -ffp-contract=off and
+// correctly rounded division/sqrt), so masks and fused counts are bit-exact
+// with a sequential CPU loop over the same expressions. Two fixed rules make
+// the result deterministic where the reference races:
+//   * scatter collisions (several reference pixels reprojecting to one output
+//     pixel, fusion.cu:164-171): the highest reference pixel index wins, via
+//     an atomicMax on the pixel index followed by a second pass that lets
+//     only the winner write;
+//   * masks[ref] is read from a copy taken before any mask of the view is
+//     written.
+// The angle test acos(dot) < theta is evaluated as dot > cos(theta), with a
+// dot outside [-1, 1] (or NaN) passing as acosf's NaN does in get_angle_cu
+// (fusion.cu:50-58).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "abi_common.h"
+
+#pragma clang fp contract(off)
+
+namespace gmvs {
+namespace {
+
+struct FuseView {
+    gmvs_camera cam;
+    const float* depth;
+    const float* normal;
+    int32_t global_id;
+};
+
+struct F3 {
+    float x, y, z;
+};
+
+__device__ inline F3 fz_world_point(float x, float y, float depth, const gmvs_camera& c) {
+    F3 p;
+    p.x = depth * (x - c.K[2]) / c.K[0];
+    p.y = depth * (y - c.K[5]) / c.K[4];
+    p.z = depth;
+    F3 t;
+    t.x = c.R[0] * p.x + c.R[3] * p.y + c.R[6] * p.z;
+    t.y = c.R[1] * p.x + c.R[4] * p.y + c.R[7] * p.z;
+    t.z = c.R[2] * p.x + c.R[5] * p.y + c.R[8] * p.z;
+    F3 C;
+    C.x = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
+    C.y = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
+    C.z = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
+    F3 o;
+    o.x = t.x + C.x;
+    o.y = t.y + C.y;
+    o.z = t.z + C.z;
+    return o;
+}
+
+__device__ inline void fz_project(const F3& X, const gmvs_camera& c, float& u, float& v, float& d) {
+    const float t0 = c.R[0] * X.x + c.R[1] * X.y + c.R[2] * X.z + c.t[0];
+    const float t1 = c.R[3] * X.x + c.R[4] * X.y + c.R[5] * X.z + c.t[1];
+    const float t2 = c.R[6] * X.x + c.R[7] * X.y + c.R[8] * X.z + c.t[2];
+    d = c.K[6] * t0 + c.K[7] * t1 + c.K[8] * t2;
+    u = (c.K[0] * t0 + c.K[1] * t1 + c.K[2] * t2) / d;
+    v = (c.K[3] * t0 + c.K[4] * t1 + c.K[5] * t2) / d;
+}
+
+// int(x + 0.5f) in [0, n): truncation toward zero (fusion.cu:112-113).
+__device__ inline bool fz_round_in(float x, int n, int* out) {
+    const float c = x + 0.5f;
+    if (!(c > -1.0f && c < (float)n)) return false;
+    *out = (int)c;
+    return true;
+}
+
+__global__ void fuse_write_view(FuseView v, FuseView* __restrict__ dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
+}
+
+__global__ void fuse_fill(int hw, float* __restrict__ depth, float* __restrict__ normal,
+                          int32_t* __restrict__ winner, const uint8_t* __restrict__ mask_src,
+                          uint8_t* __restrict__ mask_copy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hw) return;
+    depth[i] = 0.f;
+    normal[3 * i] = 0.f;
+    normal[3 * i + 1] = 0.f;
+    normal[3 * i + 2] = 0.f;
+    winner[i] = -1;
+    mask_copy[i] = mask_src[i];
+}
+
+// Pass 1: per reference pixel, the consistency check against every source,
+// the averaged point/normal, its reprojection and the mask updates.
+__global__ __launch_bounds__(256) void fuse_check(const FuseView* __restrict__ views, int nsrc,
+                                                  int h, int w, int geom_consistent,
+                                                  float reproj_px, float rel_depth,
+                                                  float cos_angle,
+                                                  const uint8_t* __restrict__ ref_mask,
+                                                  uint8_t* __restrict__ masks,
+                                                  int32_t* __restrict__ winner,
+                                                  int32_t* __restrict__ rec_idx,
+                                                  float4* __restrict__ rec_val) {
+    const int ri = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ri >= h * w) return;
+    rec_idx[ri] = -1;
+    if (ref_mask[ri]) return;
+    const gmvs_camera& rc = views[0].cam;
+    const float rd = views[0].depth[ri];
+    const F3 rn{views[0].normal[3 * ri], views[0].normal[3 * ri + 1], views[0].normal[3 * ri + 2]};
+    if (rd <= 0.0f) return;
+    const int r = ri / w, cc = ri - r * w;
+    const F3 rp = fz_world_point((float)cc, (float)r, rd, rc);
+    F3 cp = rp, cn = rn;
+    int num = 0;
+    uint32_t used_mask = 0;
+    for (int j = 0; j < nsrc; ++j) {
+        const FuseView& sv = views[j + 1];
+        float u, v, pd;
+        fz_project(rp, sv.cam, u, v, pd);
+        int sr, scol;
+        if (!fz_round_in(v, h, &sr) || !fz_round_in(u, w, &scol)) continue;
+        const int si = sr * w + scol;
+        const float sd = sv.depth[si];
+        const F3 sn{sv.normal[3 * si], sv.normal[3 * si + 1], sv.normal[3 * si + 2]};
+        if (sd <= 0.0f) continue;
+        const F3 sp = fz_world_point((float)scol, (float)sr, sd, sv.cam);
+        float tu, tv;
+        fz_project(sp, rc, tu, tv, pd);
+        const float dx = (float)cc - tu;
+        const float dy = (float)r - tv;
+        const float err = sqrtf(dx * dx + dy * dy);
+        const float rel = fabsf(pd - rd) / rd;
+        const float dot = rn.x * sn.x + rn.y * sn.y + rn.z * sn.z;
+        const bool ang_ok = !(dot >= -1.0f && dot <= 1.0f) || dot > cos_angle;
+        if (err < reproj_px && rel < rel_depth && ang_ok) {
+            cp.x += sp.x;
+            cp.y += sp.y;
+            cp.z += sp.z;
+            cn.x += sn.x;
+            cn.y += sn.y;
+            cn.z += sn.z;
+            used_mask |= 1u << j;
+            ++num;
+        }
+    }
+    if (num < geom_consistent) return;
+    const float den = (float)num + 1.0f;
+    cp.x /= den;
+    cp.y /= den;
+    cp.z /= den;
+    cn.x /= den;
+    cn.y /= den;
+    cn.z /= den;
+    float pu, pv, pdep;
+    fz_project(cp, rc, pu, pv, pdep);
+    int pr, pc;
+    if (fz_round_in(pv, h, &pr) && fz_round_in(pu, w, &pc) && pdep > 0.001f) {
+        const int pi = pr * w + pc;
+        atomicMax(&winner[pi], ri);
+        rec_idx[ri] = pi;
+        rec_val[ri] = make_float4(pdep, cn.x, cn.y, cn.z);
+    }
+    // mark the used source pixels (recomputed: the same arithmetic as above)
+    const size_t hw = (size_t)h * w;
+    for (int j = 0; j < nsrc; ++j) {
+        if (!((used_mask >> j) & 1u)) continue;
+        const FuseView& sv = views[j + 1];
+        float u, v, pd;
+        fz_project(rp, sv.cam, u, v, pd);
+        int sr, scol;
+        fz_round_in(v, h, &sr);
+        fz_round_in(u, w, &scol);
+        masks[(size_t)sv.global_id * hw + (size_t)sr * w + scol] = 1;
+    }
+}
+
+// Pass 2: the winning (highest-index) reference pixel writes each output pixel.
+__global__ void fuse_scatter(int hw, const int32_t* __restrict__ winner,
+                             const int32_t* __restrict__ rec_idx,
+                             const float4* __restrict__ rec_val, float* __restrict__ depth,
+                             float* __restrict__ normal) {
+    const int ri = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ri >= hw) return;
+    const int pi = rec_idx[ri];
+    if (pi < 0 || winner[pi] != ri) return;
+    const float4 v = rec_val[ri];
+    depth[pi] = v.x;
+    normal[3 * pi] = v.y;
+    normal[3 * pi + 1] = v.z;
+    normal[3 * pi + 2] = v.w;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct FuseLayout {
+    size_t views, mask_copy, winner, rec_idx, rec_val, total;
+};
+
+FuseLayout fuse_layout(int h, int w) {
+    const size_t hw = size_t(h) * w;
+    FuseLayout L;
+    size_t o = 0;
+    L.views = o;
+    o = align_up(o + sizeof(FuseView) * (GMVS_MAX_NGB + 1));
+    L.mask_copy = o;
+    o = align_up(o + hw);
+    L.winner = o;
+    o = align_up(o + 4 * hw);
+    L.rec_idx = o;
+    o = align_up(o + 4 * hw);
+    L.rec_val = o;
+    o = align_up(o + 16 * hw);
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+}  // namespace gmvs
+
+using namespace gmvs;
+
+extern "C" size_t gmvs_fuse_workspace_bytes(int32_t height, int32_t width) {
+    if (height <= 0 || width <= 0) return 0;
+    return fuse_layout(height, width).total;
+}
+
+extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
+                              const float* const* depths, const float* const* normals,
+                              uint8_t* masks, int32_t num_mask_views, int32_t ref_id,
+                              const int32_t* src_ids,
+                              int32_t geom_consistent, float reproj_px, float rel_depth,
+                              float cos_angle, float* out_depth, float* out_normal,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+    clear_error();
+    GMVS_ARG_CHECK(cameras && depths && normals && masks && out_depth && out_normal,
+                   "fuse_view: NULL argument");
+    GMVS_ARG_CHECK(num_images >= 1 && num_images <= GMVS_MAX_NGB + 1,
+                   "fuse_view: num_images must be in [1, %d] (got %d)", GMVS_MAX_NGB + 1,
+                   num_images);
+    GMVS_ARG_CHECK(num_images == 1 || src_ids, "fuse_view: src_ids is NULL");
+    GMVS_ARG_CHECK(ref_id >= 0 && ref_id < num_mask_views,
+                   "fuse_view: ref_id %d outside the %d mask views", ref_id, num_mask_views);
+    const int h = cameras[0].height, w = cameras[0].width;
+    GMVS_ARG_CHECK(h > 0 && w > 0, "fuse_view: bad image size %dx%d", w, h);
+    for (int i = 0; i < num_images; ++i) {
+        GMVS_ARG_CHECK(depths[i] && normals[i], "fuse_view: depths/normals[%d] is NULL", i);
+        GMVS_ARG_CHECK(cameras[i].height == h && cameras[i].width == w,
+                       "fuse_view: view %d is %dx%d, reference %dx%d", i, cameras[i].width,
+                       cameras[i].height, w, h);
+        if (i)
+            GMVS_ARG_CHECK(src_ids[i - 1] >= 0 && src_ids[i - 1] < num_mask_views,
+                           "fuse_view: src id %d outside the %d mask views", src_ids[i - 1],
+                           num_mask_views);
+    }
+    const FuseLayout L = fuse_layout(h, w);
+    GMVS_ARG_CHECK(workspace, "fuse_view: workspace is NULL");
+    if (workspace_bytes < L.total) {
+        set_error("fuse_view: workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
+        return GMVS_ERR_NOSPACE;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char* ws = reinterpret_cast<char*>(workspace);
+    FuseView* vtab = reinterpret_cast<FuseView*>(ws + L.views);
+    for (int i = 0; i < num_images; ++i) {
+        FuseView fv;
+        fv.cam = cameras[i];
+        fv.depth = depths[i];
+        fv.normal = normals[i];
+        fv.global_id = i ? src_ids[i - 1] : ref_id;
+        hipLaunchKernelGGL(fuse_write_view, dim3(1), dim3(64), 0, st, fv, vtab + i);
+        GMVS_LAUNCH_CHECK();
+    }
+    const int hw = h * w;
+    const dim3 grid((hw + 255) / 256), block(256);
+    uint8_t* mask_copy = reinterpret_cast<uint8_t*>(ws + L.mask_copy);
+    int32_t* winner = reinterpret_cast<int32_t*>(ws + L.winner);
+    int32_t* rec_idx = reinterpret_cast<int32_t*>(ws + L.rec_idx);
+    float4* rec_val = reinterpret_cast<float4*>(ws + L.rec_val);
+    hipLaunchKernelGGL(fuse_fill, grid, block, 0, st, hw, out_depth, out_normal, winner,
+                       masks + (size_t)ref_id * hw, mask_copy);
+    GMVS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(fuse_check, grid, block, 0, st,
+                       reinterpret_cast<const FuseView*>(ws + L.views), num_images - 1, h, w,
+                       geom_consistent, reproj_px, rel_depth, cos_angle, mask_copy, masks, winner,
+                       rec_idx, rec_val);
+    GMVS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(fuse_scatter, grid, block, 0, st, hw, winner, rec_idx, rec_val, out_depth,
+                       out_normal);
+    GMVS_LAUNCH_CHECK();
+    return GMVS_OK;
+}

@@ -1,0 +1,363 @@
+// Device helpers of the PatchMatch kernels (gfx950): counter-based RNG,
+// camera-frame geometry, view descriptors and the hoisted bilateral NCC.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gmvs_amd.h"
+
+namespace gmvs {
+
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr int kMaxViews = 31;  // sources per reference (32-bit view masks)
+
+// ------------------------------------------------------------------ RNG ---
+// Philox4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64 products
+// with the published multipliers, key bumped by the Weyl constants between
+// rounds. Counter = (pixel, phase, slot / 4, 0), key = (seed, view); a draw
+// is word slot % 4 of that block, so every random use has a fixed slot.
+__host__ __device__ inline void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                             uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// Uniform in (0, 1], exact in FP32: ((x >> 8) + 1) * 2^-24.
+__host__ __device__ inline float u01(uint32_t x) {
+    return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// Fixed draw slots (identical in the oracle).
+enum : uint32_t {
+    RNG_PHASE_INIT = 0x40000000u,
+    RNG_INIT_DEPTH = 0,
+    RNG_INIT_ANGLE = 1,
+    RNG_INIT_NORMAL = 4,
+    RNG_VIEW = 0,
+    RNG_DEPTH_RAND = 15,
+    RNG_DEPTH_PERT = 16,
+    RNG_PERT_ANGLE = 17,
+    RNG_PRIOR_ANGLE = 20,
+    RNG_NORMAL = 24,
+    RNG_REJECT_TRIES = 16,
+};
+
+struct Rng {
+    uint32_t pixel, phase, k0, k1;
+    __device__ void block(uint32_t b, uint32_t out[4]) const {
+        philox_block(pixel, phase, b, 0u, k0, k1, out);
+    }
+    __device__ float u(uint32_t slot) const {
+        uint32_t o[4];
+        block(slot >> 2, o);
+        return u01(o[slot & 3u]);
+    }
+};
+
+// ------------------------------------------------------------- geometry ---
+struct RefCam {
+    float K[9], R[9], t[3], C[3];
+    int h, w;
+};
+
+// One source view of one reference view. A = Ks R_rel Kr^-1 and b = Ks t_rel
+// are composed in FP64 on the device (pm_setup_views) so that the
+// per-hypothesis homography is the FP32 rank-1 update H = A - b m^T with
+// m^T = n^T Kr^-1 / d (the factorisation of compute_homography, src 211-296).
+struct ViewDesc {
+    float A[9];
+    float b[3];
+    float K[9], R[9], t[3], C[3];  // source camera for the geometric term
+    const float4* quad;            // packed bilinear cells [H*W]
+    const float* depth;            // source depth map [H*W] or nullptr
+};
+
+__device__ inline float depth_from_plane(const RefCam& c, float4 pl, float px, float py) {
+    return -pl.w * c.K[0] /
+           ((px - c.K[2]) * pl.x + (c.K[0] / c.K[4]) * (py - c.K[5]) * pl.y + c.K[0] * pl.z);
+}
+
+__device__ inline float dist_to_origin(const RefCam& c, float px, float py, float depth, float4 n) {
+    const float X0 = depth * (px - c.K[2]) / c.K[0];
+    const float X1 = depth * (py - c.K[5]) / c.K[4];
+    return -(n.x * X0 + n.y * X1 + n.z * depth);
+}
+
+__device__ inline void view_dir(const RefCam& c, float px, float py, float depth, float v[3]) {
+    const float X0 = depth * (px - c.K[2]) / c.K[0];
+    const float X1 = depth * (py - c.K[5]) / c.K[4];
+    const float X2 = depth;
+    const float nrm = sqrtf(X0 * X0 + X1 * X1 + X2 * X2);
+    v[0] = X0 / nrm;
+    v[1] = X1 / nrm;
+    v[2] = X2 / nrm;
+}
+
+__device__ inline void normalize3(float4& v) {
+    const float s = 1.0f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.x *= s;
+    v.y *= s;
+    v.z *= s;
+}
+
+// Random unit normal facing the camera (Marsaglia, A16) from fixed slots.
+__device__ inline float4 random_normal(const RefCam& c, float px, float py, const Rng& rng,
+                                       uint32_t slot0, float depth) {
+    float q1 = 0.f, q2 = 0.f, s = 2.f;
+    bool ok = false;
+    for (uint32_t a = 0; a < RNG_REJECT_TRIES && !ok; a += 2) {
+        // two attempts per Philox block (slots slot0 + 2a .. slot0 + 2a + 3)
+        uint32_t o[4];
+        rng.block((slot0 + 2 * a) >> 2, o);
+        const float a1 = 2.0f * u01(o[0]) - 1.0f, b1 = 2.0f * u01(o[1]) - 1.0f;
+        const float s1 = a1 * a1 + b1 * b1;
+        if (s1 < 1.0f) {
+            q1 = a1;
+            q2 = b1;
+            s = s1;
+            ok = true;
+        } else {
+            const float a2 = 2.0f * u01(o[2]) - 1.0f, b2 = 2.0f * u01(o[3]) - 1.0f;
+            const float s2 = a2 * a2 + b2 * b2;
+            if (s2 < 1.0f) {
+                q1 = a2;
+                q2 = b2;
+                s = s2;
+                ok = true;
+            }
+        }
+    }
+    if (!ok) {
+        uint32_t o[4];
+        rng.block(slot0 >> 2, o);
+        q1 = 2.0f * u01(o[0]) - 1.0f;
+        q2 = 2.0f * u01(o[1]) - 1.0f;
+        const float r = sqrtf(q1 * q1 + q2 * q2);
+        if (r > 0.f) {
+            q1 = q1 / r * 0.5f;
+            q2 = q2 / r * 0.5f;
+        } else {
+            q1 = 0.5f;
+            q2 = 0.f;
+        }
+        s = q1 * q1 + q2 * q2;
+    }
+    const float sq = sqrtf(1.0f - s);
+    float4 n = make_float4(2.0f * q1 * sq, 2.0f * q2 * sq, 1.0f - 2.0f * s, 0.f);
+    float vd[3];
+    view_dir(c, px, py, depth, vd);
+    if (n.x * vd[0] + n.y * vd[1] + n.z * vd[2] > 0.f) {
+        n.x = -n.x;
+        n.y = -n.y;
+        n.z = -n.z;
+    }
+    normalize3(n);
+    return n;
+}
+
+// Euler-angle perturbation of a normal (A17) from three fixed slots.
+__device__ inline float4 perturbed_normal(const RefCam& c, float px, float py, float4 n,
+                                          float u1, float u2, float u3, float pert) {
+    float vd[3];
+    view_dir(c, px, py, 1.0f, vd);
+    const float a1 = (u1 - 0.5f) * pert, a2 = (u2 - 0.5f) * pert, a3 = (u3 - 0.5f) * pert;
+    float s1, c1, s2, c2, s3, c3;
+    sincosf(a1, &s1, &c1);
+    sincosf(a2, &s2, &c2);
+    sincosf(a3, &s3, &c3);
+    const float R0 = c2 * c3, R1 = c3 * s1 * s2 - c1 * s3, R2 = s1 * s3 + c1 * c3 * s2;
+    const float R3 = c2 * s3, R4 = c1 * c3 + s1 * s2 * s3, R5 = c1 * s2 * s3 - c3 * s1;
+    const float R6 = -s2, R7 = c2 * s1, R8 = c1 * c2;
+    float4 o = make_float4(R0 * n.x + R1 * n.y + R2 * n.z, R3 * n.x + R4 * n.y + R5 * n.z,
+                           R6 * n.x + R7 * n.y + R8 * n.z, n.w);
+    if (o.x * vd[0] + o.y * vd[1] + o.z * vd[2] >= 0.f) o = n;
+    normalize3(o);
+    return o;
+}
+
+// m = n^T Kr^-1 / d for the rank-1 homography update.
+__device__ inline void plane_m(const RefCam& c, float4 pl, float m[3]) {
+    const float id = 1.0f / pl.w;
+    const float a = pl.x / c.K[0], b = pl.y / c.K[4];
+    m[0] = a * id;
+    m[1] = b * id;
+    m[2] = (pl.z - a * c.K[2] - b * c.K[5]) * id;
+}
+
+__device__ inline void homography(const float* __restrict__ A, const float* __restrict__ b,
+                                  const float m[3], float H[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) H[3 * i + j] = fmaf(-b[i], m[j], A[3 * i + j]);
+}
+
+// Software bilinear of an f32 map with the reference's index rule
+// (src 10-33): index = clamp(trunc(c), 0, n-1), fraction = c - index.
+__device__ inline float bilinear_map(const float* __restrict__ img, float x, float y, int h,
+                                     int w) {
+    const float xf = fminf(fmaxf(floorf(x), 0.f), (float)(w - 1));
+    const float yf = fminf(fmaxf(floorf(y), 0.f), (float)(h - 1));
+    const int wl = (int)xf, hl = (int)yf;
+    const float fx = x - xf, fy = y - yf;
+    const int wu = (wl == w - 1) ? wl : wl + 1;
+    const int hu = (hl == h - 1) ? hl : hl + 1;
+    const float v0 = img[hl * w + wl], v1 = img[hl * w + wu];
+    const float v2 = img[hu * w + wl], v3 = img[hu * w + wu];
+    const float a = fmaf(fx, v1 - v0, v0);
+    const float bb = fmaf(fx, v3 - v2, v2);
+    return fmaf(fy, bb - a, a);
+}
+
+__device__ inline void world_point(float x, float y, float depth, const float* K, const float* R,
+                                   const float* C, float X[3]) {
+    const float p0 = depth * (x - K[2]) / K[0];
+    const float p1 = depth * (y - K[5]) / K[4];
+    const float p2 = depth;
+    X[0] = R[0] * p0 + R[3] * p1 + R[6] * p2 + C[0];
+    X[1] = R[1] * p0 + R[4] * p1 + R[7] * p2 + C[1];
+    X[2] = R[2] * p0 + R[5] * p1 + R[8] * p2 + C[2];
+}
+
+__device__ inline void project(const float X[3], const float* K, const float* R, const float* t,
+                               float& u, float& v) {
+    const float t0 = R[0] * X[0] + R[1] * X[1] + R[2] * X[2] + t[0];
+    const float t1 = R[3] * X[0] + R[4] * X[1] + R[5] * X[2] + t[1];
+    const float t2 = R[6] * X[0] + R[7] * X[1] + R[8] * X[2] + t[2];
+    const float d = K[6] * t0 + K[7] * t1 + K[8] * t2;
+    u = (K[0] * t0 + K[1] * t1 + K[2] * t2) / d;
+    v = (K[3] * t0 + K[4] * t1 + K[5] * t2) / d;
+}
+
+// Forward-backward reprojection error against a source depth map, capped at
+// 5 px (compute_geom_consistency_cost, src 562-593).
+__device__ inline float geom_cost(const RefCam& rc, const ViewDesc& vd, float4 pl, float px,
+                                  float py) {
+    const float depth = depth_from_plane(rc, pl, px, py);
+    float X[3];
+    world_point(px, py, depth, rc.K, rc.R, rc.C, X);
+    float su, sv;
+    project(X, vd.K, vd.R, vd.t, su, sv);
+    const float sd = bilinear_map(vd.depth, su, sv, rc.h, rc.w);
+    if (sd == 0.0f) return 5.0f;
+    float Y[3];
+    world_point(su, sv, sd, vd.K, vd.R, vd.C, Y);
+    float bu, bv;
+    project(Y, rc.K, rc.R, rc.t, bu, bv);
+    const float dc = px - bu, dr = py - bv;
+    return fminf(5.0f, sqrtf(dc * dc + dr * dr));
+}
+
+// ------------------------------------------------------------------ NCC ---
+// Per-pixel hoist of the reference side of the bilateral NCC
+// (compute_bilateral_NCC, src 362-445): the NT*NT taps' weights w and
+// w * (r - c) with c the centre intensity, and the weighted mean/variance of
+// the reference patch from shifted (centre-relative) moments.
+template <int NT>
+struct Hoist {
+    float wt[NT * NT];
+    float wr[NT * NT];
+    float tx[NT];  // clamped tap columns (reference taps, src 400-405)
+    float ty[NT];  // clamped tap rows
+    float inv_w, mean_r, var_r, c;
+    float px, py;
+};
+
+template <int NT>
+__device__ inline void hoist_ref(Hoist<NT>& hs, const float* __restrict__ ref, int px, int py,
+                                 int h, int w, int radius, int inc, float sigma_spatial,
+                                 float sigma_color) {
+    hs.px = (float)px;
+    hs.py = (float)py;
+    const float c = ref[py * w + px];
+    hs.c = c;
+    const float ss = 2.0f * sigma_spatial * sigma_spatial;
+    const float sc = 2.0f * sigma_color * sigma_color;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        hs.tx[i] = (float)min(max(px - radius + i * inc, 0), w - 1);
+        hs.ty[i] = (float)min(max(py - radius + i * inc, 0), h - 1);
+    }
+    float W = 0.f, R1 = 0.f, R2 = 0.f;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const float di = (float)(-radius + i * inc);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const float dj = (float)(-radius + j * inc);
+            const float r = ref[(int)hs.ty[j] * w + (int)hs.tx[i]] - c;
+            const float wgt = expf(-sqrtf(di * di + dj * dj) / ss - fabsf(r) / sc);
+            hs.wt[i * NT + j] = wgt;
+            hs.wr[i * NT + j] = wgt * r;
+            W += wgt;
+            R1 = fmaf(wgt, r, R1);
+            R2 = fmaf(wgt * r, r, R2);
+        }
+    }
+    hs.inv_w = 1.0f / W;
+    hs.mean_r = R1 * hs.inv_w;
+    hs.var_r = R2 * hs.inv_w - hs.mean_r * hs.mean_r;
+}
+
+// One bilateral NCC of the hoisted pixel against one source view. The
+// source image is stored as packed bilinear cells (v00, v01-v00, v10-v00,
+// v11-v10-v01+v00) with the reference's edge clamping already applied, so a
+// tap is one 16-byte gather and three FMAs.
+template <int NT>
+__device__ inline float ncc_eval(const Hoist<NT>& hs, const float H[9],
+                                 const float4* __restrict__ quad, int h, int w) {
+    const float cz = fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8]));
+    const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) / cz;
+    const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) / cz;
+    if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
+    const float wm1 = (float)(w - 1), hm1 = (float)(h - 1);
+    float S1 = 0.f, S2 = 0.f, X = 0.f;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const float au = fmaf(H[0], hs.tx[i], H[2]);
+        const float av = fmaf(H[3], hs.tx[i], H[5]);
+        const float az = fmaf(H[6], hs.tx[i], H[8]);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const float u = fmaf(H[1], hs.ty[j], au);
+            const float v = fmaf(H[4], hs.ty[j], av);
+            const float z = fmaf(H[7], hs.ty[j], az);
+            const float rz = __builtin_amdgcn_rcpf(z);
+            const float x = u * rz, y = v * rz;
+            const float xf = fminf(fmaxf(floorf(x), 0.f), wm1);
+            const float yf = fminf(fmaxf(floorf(y), 0.f), hm1);
+            const float fx = x - xf, fy = y - yf;
+            const float4 q = quad[(int)yf * w + (int)xf];
+            const float top = fmaf(fx, q.y, q.x);
+            const float dyv = fmaf(fx, q.w, q.z);
+            const float s = fmaf(fy, dyv, top) - hs.c;
+            const float ws = hs.wt[i * NT + j] * s;
+            S1 += ws;
+            S2 = fmaf(ws, s, S2);
+            X = fmaf(hs.wr[i * NT + j], s, X);
+        }
+    }
+    const float ms = S1 * hs.inv_w;
+    const float var_s = S2 * hs.inv_w - ms * ms;
+    if (hs.var_r < 1e-5f || var_s < 1e-5f) return 2.0f;
+    const float cov = X * hs.inv_w - hs.mean_r * ms;
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / sqrtf(hs.var_r * var_s)));
+}
+
+}  // namespace gmvs

@@ -1,0 +1,1124 @@
+// PatchMatch MVS kernels for gfx950 and their C-ABI drivers.
+//
+// One half-sweep of the red/black checkerboard (black_red_pixel_update,
+// gmvs/src/patch_match.cu:1085-1142; Black/RedPixelUpdate,
+// gmvs/lib/src/patch_match_kernel.cu:1579-1653) runs as two kernels:
+//   pm_candidates     -- adaptive candidate search over the state as it was at
+//                        the start of the half-sweep, plus the 8 x V NCC cost
+//                        table the view selection needs. It writes nothing
+//                        to the state, only a per-pixel record (costs, the
+//                        candidate planes, the flags) into the workspace;
+//   pm_select_refine  -- joint view selection, candidate adoption (plain or
+//                        planar-prior scoring, optional geometric term) and
+//                        random refinement, costed over the selected views
+//                        only; each thread writes its own pixel.
+// Because pm_select_refine reads neighbour planes only from the records,
+// every read of the half-sweep sees the snapshot (the reference reads
+// same-colour pixels that other threads are rewriting, lib 978-1104).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <vector>
+
+#include "abi_common.h"
+#include "pm_device.h"
+
+namespace gmvs {
+
+constexpr int kTile = 16;  // 16 x 16 threads per block
+constexpr int kBlock = kTile * kTile;
+constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+struct PMConst {
+    RefCam rc;
+    const float* ref;             // reference intensities [H*W]
+    const ViewDesc* views;        // [nv]
+    const float* init_depth;      // INIT_DEPTH_NORMAL
+    const float* init_normal;     // INIT_DEPTH_NORMAL, world frame
+    const float4* prior_planes;   // planar prior
+    const uint32_t* plane_masks;  // planar prior
+    int nv, h, w, wh;             // wh = active columns per row
+    int radius, inc;
+    float sigma_s, sigma_c;
+    float dmin, dmax;
+    int top_k;
+    uint32_t seed, view;
+    int init_mode, geom, prior;
+};
+
+// Block -> tile with the tiles of one XCD contiguous, so that neighbouring
+// tiles (which gather from the same source-image region) share an L2.
+__device__ inline bool tile_of_block(int ntiles, int& tile) {
+    const int chunk = (ntiles + kXcds - 1) / kXcds;
+    const int b = blockIdx.x;
+    tile = (b % kXcds) * chunk + b / kXcds;
+    return tile < ntiles;
+}
+
+__host__ inline int grid_for_tiles(int ntiles) {
+    const int chunk = (ntiles + kXcds - 1) / kXcds;
+    return chunk * kXcds;
+}
+
+// ------------------------------------------------------------ setup ---
+// FP64 per-view composition of the homography factors:
+// A = Ks R_rel Kr^-1, b = Ks t_rel with R_rel = Rs Rr^T,
+// t_rel = Rs (C_r - C_s), C = -R^T t (compute_homography, src 211-296; as
+// there, Kr^-1 uses fx, fy, cx, cy and Ks contributes K0, K2, K4, K5, K8).
+// One single-thread launch per source view; the cameras travel as kernel
+// arguments.
+__global__ void pm_setup_view(gmvs_camera r, gmvs_camera s, const float4* quad,
+                              const float* depth, ViewDesc* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double Cr[3], Cs[3];
+    for (int i = 0; i < 3; ++i) {
+        Cr[i] = -((double)r.R[i] * r.t[0] + (double)r.R[3 + i] * r.t[1] + (double)r.R[6 + i] * r.t[2]);
+        Cs[i] = -((double)s.R[i] * s.t[0] + (double)s.R[3 + i] * s.t[1] + (double)s.R[6 + i] * s.t[2]);
+    }
+    double Rrel[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            Rrel[3 * i + j] = (double)s.R[3 * i] * r.R[3 * j] + (double)s.R[3 * i + 1] * r.R[3 * j + 1] +
+                              (double)s.R[3 * i + 2] * r.R[3 * j + 2];
+    double trel[3];
+    for (int i = 0; i < 3; ++i)
+        trel[i] = (double)s.R[3 * i] * (Cr[0] - Cs[0]) + (double)s.R[3 * i + 1] * (Cr[1] - Cs[1]) +
+                  (double)s.R[3 * i + 2] * (Cr[2] - Cs[2]);
+    const double fx = r.K[0], fy = r.K[4], cx = r.K[2], cy = r.K[5];
+    const double Ki[9] = {1.0 / fx, 0.0, -cx / fx, 0.0, 1.0 / fy, -cy / fy, 0.0, 0.0, 1.0};
+    const double Ks[9] = {s.K[0], 0.0, s.K[2], 0.0, s.K[4], s.K[5], 0.0, 0.0, s.K[8]};
+    double M[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            M[3 * i + j] = Rrel[3 * i] * Ki[j] + Rrel[3 * i + 1] * Ki[3 + j] + Rrel[3 * i + 2] * Ki[6 + j];
+    ViewDesc d;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j)
+            d.A[3 * i + j] = (float)(Ks[3 * i] * M[j] + Ks[3 * i + 1] * M[3 + j] + Ks[3 * i + 2] * M[6 + j]);
+        d.b[i] = (float)(Ks[3 * i] * trel[0] + Ks[3 * i + 1] * trel[1] + Ks[3 * i + 2] * trel[2]);
+    }
+    for (int i = 0; i < 9; ++i) {
+        d.K[i] = s.K[i];
+        d.R[i] = s.R[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        d.t[i] = s.t[i];
+        d.C[i] = -(s.R[i] * s.t[0] + s.R[3 + i] * s.t[1] + s.R[6 + i] * s.t[2]);
+    }
+    d.quad = quad;
+    d.depth = depth;
+    *out = d;
+}
+
+// Packs an f32 image into bilinear cells (v00, v01-v00, v10-v00,
+// v11-v10-v01+v00), with the right/bottom neighbours clamped to the last
+// column/row as the reference's software bilinear does (src 20-21).
+__global__ void pm_pack_quads(const float* __restrict__ img, float4* __restrict__ quad, int h,
+                              int w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h * w) return;
+    const int y = i / w, x = i - y * w;
+    const int xu = (x == w - 1) ? x : x + 1;
+    const int yu = (y == h - 1) ? y : y + 1;
+    const float v0 = img[y * w + x], v1 = img[y * w + xu];
+    const float v2 = img[yu * w + x], v3 = img[yu * w + xu];
+    quad[i] = make_float4(v0, v1 - v0, v2 - v0, (v3 - v2) - (v1 - v0));
+}
+
+// ---------------------------------------------------------------- init ---
+template <int NT>
+__global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict__ planes,
+                                                  float* __restrict__ costs,
+                                                  uint32_t* __restrict__ sel) {
+    __shared__ float s_c[kMaxViews][kBlock];
+    int tile;
+    if (!tile_of_block(((P.w + kTile - 1) / kTile) * ((P.h + kTile - 1) / kTile), tile)) return;
+    const int tx = (P.w + kTile - 1) / kTile;
+    const int px = (tile % tx) * kTile + (threadIdx.x & (kTile - 1));
+    const int py = (tile / tx) * kTile + (threadIdx.x / kTile);
+    if (px >= P.w || py >= P.h) return;
+    const int c = py * P.w + px;
+    const RefCam& rc = P.rc;
+    const float fpx = (float)px, fpy = (float)py;
+    Rng rng{(uint32_t)c, RNG_PHASE_INIT, P.seed, P.view};
+
+    float4 pl;
+    if (P.init_mode == GMVS_INIT_DEPTH_NORMAL) {
+        const float n0 = P.init_normal[3 * c], n1 = P.init_normal[3 * c + 1], n2 = P.init_normal[3 * c + 2];
+        pl.x = rc.R[0] * n0 + rc.R[1] * n1 + rc.R[2] * n2;
+        pl.y = rc.R[3] * n0 + rc.R[4] * n1 + rc.R[5] * n2;
+        pl.z = rc.R[6] * n0 + rc.R[7] * n1 + rc.R[8] * n2;
+        pl.w = dist_to_origin(rc, fpx, fpy, P.init_depth[c], pl);
+    } else if (P.init_mode == GMVS_INIT_PRIOR) {
+        const float4 st = planes[c];
+        if (P.prior && P.plane_masks[c] > 0u && costs[c] >= 0.1f) {
+            const float4 pp = P.prior_planes[c];
+            const float pert = 0.02f;
+            const float dlo = (1.0f - 3.0f * pert) * pp.w;
+            const float dhi = (1.0f + 3.0f * pert) * pp.w;
+            uint32_t o[4];
+            rng.block(0, o);
+            const float dp = u01(o[RNG_INIT_DEPTH]) * (dhi - dlo) + dlo;
+            pl = perturbed_normal(rc, fpx, fpy, pp, u01(o[1]), u01(o[2]), u01(o[3]),
+                                  3.0f * pert * kPiF);
+            pl.w = dp;
+        } else {
+            pl = st;
+            pl.w = dist_to_origin(rc, fpx, fpy, st.w, st);
+        }
+    } else {
+        const float depth = rng.u(RNG_INIT_DEPTH) * (P.dmax - P.dmin) + P.dmin;
+        pl = random_normal(rc, fpx, fpy, rng, RNG_INIT_NORMAL, depth);
+        pl.w = dist_to_origin(rc, fpx, fpy, depth, pl);
+    }
+
+    Hoist<NT> hs;
+    hoist_ref<NT>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+    float m[3];
+    plane_m(rc, pl, m);
+    int valid = 0;
+    for (int v = 0; v < P.nv; ++v) {
+        const ViewDesc& vd = P.views[v];
+        float H[9];
+        homography(vd.A, vd.b, m, H);
+        const float cv = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+        s_c[v][threadIdx.x] = cv;
+        valid += (cv < 2.0f);
+    }
+    // Top-k mean of the valid costs and the view mask of costs <= the k-th
+    // smallest (compute_multi_view_initial_cost_and_selected_views,
+    // src 447-501).
+    const int k = min(valid, P.top_k);
+    uint32_t bits = 0;
+    float cost = 2.0f;
+    if (k > 0) {
+        uint32_t taken = 0;
+        float sum = 0.f, thr = 0.f;
+        for (int r = 0; r < k; ++r) {
+            float best = 0.f;
+            int bi = -1;
+            for (int v = 0; v < P.nv; ++v) {
+                if ((taken >> v) & 1u) continue;
+                const float x = s_c[v][threadIdx.x];
+                if (bi < 0 || x < best) {
+                    best = x;
+                    bi = v;
+                }
+            }
+            taken |= 1u << bi;
+            sum += best;
+            thr = best;
+        }
+        for (int v = 0; v < P.nv; ++v)
+            if (s_c[v][threadIdx.x] <= thr) bits |= 1u << v;
+        cost = sum / (float)k;
+    }
+    planes[c] = pl;
+    costs[c] = cost;
+    sel[c] = bits;
+}
+
+// ------------------------------------------------------- half-sweep, A ---
+struct Active {
+    int px, py, aidx;
+    bool ok;
+};
+
+__device__ inline Active active_pixel(const PMConst& P, int tile, int color) {
+    const int tx = (P.wh + kTile - 1) / kTile;
+    const int ax = (tile % tx) * kTile + (threadIdx.x & (kTile - 1));
+    const int y = (tile / tx) * kTile + (threadIdx.x / kTile);
+    Active a;
+    a.py = y;
+    a.px = 2 * ax + ((y + color) & 1);
+    a.aidx = y * P.wh + ax;
+    a.ok = (y < P.h) && (ax < P.wh) && (a.px < P.w);
+    return a;
+}
+
+// Adaptive checkerboard candidate search (checkerboard_propagation,
+// src 720-913). Order: 0 up_near, 1 up_far, 2 down_near, 3 down_far,
+// 4 left_near, 5 left_far, 6 right_near, 7 right_far.
+__device__ inline void candidate_search(const float* __restrict__ cs, int px, int py, int h, int w,
+                                        uint32_t& flags, int pos[8]) {
+    const int c = py * w + px;
+    flags = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pos[k] = c;
+    if (py > 2) {
+        flags |= 1u << 1;
+        int best = c - 3 * w;
+        float m = cs[best];
+        for (int i = 1; i < 11; ++i)
+            if (py > 2 + 2 * i) {
+                const int q = c - 3 * w - 2 * i * w;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+        pos[1] = best;
+    }
+    if (py < h - 3) {
+        flags |= 1u << 3;
+        int best = c + 3 * w;
+        float m = cs[best];
+        for (int i = 1; i < 11; ++i)
+            if (py < h - 3 - 2 * i) {
+                const int q = c + 3 * w + 2 * i * w;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+        pos[3] = best;
+    }
+    if (px > 2) {
+        flags |= 1u << 5;
+        int best = c - 3;
+        float m = cs[best];
+        for (int i = 1; i < 11; ++i)
+            if (px > 2 + 2 * i) {
+                const int q = c - 3 - 2 * i;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+        pos[5] = best;
+    }
+    if (px < w - 3) {
+        flags |= 1u << 7;
+        int best = c + 3;
+        float m = cs[best];
+        for (int i = 1; i < 11; ++i)
+            if (px < w - 3 - 2 * i) {
+                const int q = c + 3 + 2 * i;
+                const float x = cs[q];
+                if (m < x) {  // the reference keeps the maximum here (src 794)
+                    m = x;
+                    best = q;
+                }
+            }
+        pos[7] = best;
+    }
+    if (py > 0) {
+        flags |= 1u << 0;
+        const int base = c - w;
+        int best = base;
+        float m = cs[best];
+        for (int i = 0; i < 3; ++i) {
+            if (py > 1 + i && px > i) {
+                const int q = base - (1 + i) * w - i;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+            if (py > 1 + i && px < w - 1 - i) {
+                const int q = base - (1 + i) * w + i;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+        }
+        pos[0] = best;
+    }
+    if (py < h - 1) {
+        flags |= 1u << 2;
+        const int base = c + w;
+        int best = base;
+        float m = cs[best];
+        for (int i = 0; i < 3; ++i) {
+            if (py < h - 2 - i && px > i) {
+                const int q = base + (1 + i) * w - i;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+            if (py < h - 2 - i && px < w - 1 - i) {
+                const int q = base + (1 + i) * w + i;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+        }
+        pos[2] = best;
+    }
+    if (px > 0) {
+        flags |= 1u << 4;
+        const int base = c - 1;
+        int best = base;
+        float m = cs[best];
+        for (int i = 0; i < 3; ++i) {
+            if (px > 1 + i && py > i) {
+                const int q = base - (1 + i) - i * w;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+            if (px > 1 + i && py < h - 1 - i) {
+                const int q = base - (1 + i) + i * w;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+        }
+        pos[4] = best;
+    }
+    if (px < w - 1) {
+        flags |= 1u << 6;
+        const int base = c + 1;
+        int best = base;
+        float m = cs[best];
+        for (int i = 0; i < 3; ++i) {
+            if (px < w - 2 - i && py > i) {
+                const int q = base + (1 + i) - i * w;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+            if (px < w - 2 - i && py < h - 1 - i) {
+                const int q = base + (1 + i) + i * w;
+                const float x = cs[q];
+                if (x < m) {
+                    m = x;
+                    best = q;
+                }
+            }
+        }
+        pos[6] = best;
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
+                                                        const float4* __restrict__ planes,
+                                                        const float* __restrict__ costs,
+                                                        float* __restrict__ ca,
+                                                        float4* __restrict__ cplane,
+                                                        uint8_t* __restrict__ cflag, int NA) {
+    int tile;
+    if (!tile_of_block(((P.wh + kTile - 1) / kTile) * ((P.h + kTile - 1) / kTile), tile)) return;
+    const Active a = active_pixel(P, tile, color);
+    if (!a.ok) return;
+    uint32_t flags;
+    int pos[8];
+    candidate_search(costs, a.px, a.py, P.h, P.w, flags, pos);
+    cflag[a.aidx] = (uint8_t)flags;
+    Hoist<NT> hs;
+    hoist_ref<NT>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+    for (int k = 0; k < 8; ++k) {
+        if (!((flags >> k) & 1u)) continue;
+        const float4 pl = planes[pos[k]];
+        cplane[(size_t)k * NA + a.aidx] = pl;
+        float m[3];
+        plane_m(P.rc, pl, m);
+        for (int v = 0; v < P.nv; ++v) {
+            const ViewDesc& vd = P.views[v];
+            float H[9];
+            homography(vd.A, vd.b, m, H);
+            ca[((size_t)k * P.nv + v) * NA + a.aidx] = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+        }
+    }
+}
+
+// ------------------------------------------------------- half-sweep, B ---
+__device__ inline int argmin_last8(const float* c) {
+    float m = c[0];
+    int k = 0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+        if (c[i] <= m) {
+            m = c[i];
+            k = i;
+        }
+    return k;
+}
+
+__device__ inline int argmax_last8(const float* c) {
+    float m = c[0];
+    int k = 0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+        if (c[i] >= m) {
+            m = c[i];
+            k = i;
+        }
+    return k;
+}
+
+// Planar-prior score exp(-c^2/beta) * (gamma + exp(-dd^2/2sd^2) exp(-da^2/2sa^2))
+// (lib 1256-1277, 702-707, 784-793).
+__device__ inline float prior_score(const PMConst& P, float cost, float depth, float dprior,
+                                    float4 pp, float4 pl) {
+    const float dsig = (P.dmax - P.dmin) / 64.0f;
+    const float two_ds2 = 2.0f * dsig * dsig;
+    const float asig = kPiF * (5.0f / 180.0f);
+    const float two_as2 = 2.0f * asig * asig;
+    const float dd = depth - dprior;
+    const float ang = acosf(pp.x * pl.x + pp.y * pl.y + pp.z * pl.z);
+    const float pr = 0.5f + expf(-dd * dd / two_ds2) * expf(-ang * ang / two_as2);
+    return expf(-cost * cost / 0.18f) * pr;
+}
+
+__device__ inline float ca_entry(const float* __restrict__ ca, uint32_t flags, int k, int v, int nv,
+                                 int NA, int aidx) {
+    // cost_array[8][32] = {2.0f} leaves every entry but [0][0] at zero
+    // (lib 854, src 717); unflagged rows keep those values.
+    if ((flags >> k) & 1u) return ca[((size_t)k * nv + v) * NA + aidx];
+    return (k == 0 && v == 0) ? 2.0f : 0.0f;
+}
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void pm_select_refine(
+    PMConst P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
+    uint32_t* __restrict__ sel, const float* __restrict__ ca, const float4* __restrict__ cplane,
+    const uint8_t* __restrict__ cflag, int NA) {
+    __shared__ float s_cdf[kMaxViews][kBlock];
+    __shared__ ViewDesc s_views[kMaxViews];
+    {
+        const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
+        for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+        __syncthreads();
+    }
+    int tile;
+    if (!tile_of_block(((P.wh + kTile - 1) / kTile) * ((P.h + kTile - 1) / kTile), tile)) return;
+    const Active a = active_pixel(P, tile, color);
+    if (!a.ok) return;
+    const int px = a.px, py = a.py, aidx = a.aidx;
+    const int c = py * P.w + px;
+    const float fpx = (float)px, fpy = (float)py;
+    const RefCam& rc = P.rc;
+    const int nv = P.nv;
+    const uint32_t flags = cflag[aidx];
+    const Rng rng{(uint32_t)c, (uint32_t)iter, P.seed, P.view};
+
+    // ---- joint view selection (src 915-981)
+    uint32_t nbsel[4];
+    const int nb[4] = {c - P.w, c + P.w, c - 1, c + 1};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) nbsel[i] = ((flags >> (2 * i)) & 1u) ? sel[nb[i]] : 0u;
+    const float tau = 0.8f * expf((float)(iter * iter) / -90.0f);
+    float psum = 0.f;
+    for (int v = 0; v < nv; ++v) {
+        float pri = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if ((flags >> (2 * i)) & 1u) pri += ((nbsel[i] >> v) & 1u) ? 0.9f : 0.1f;
+        float count = 0.f, tmpw = 0.f;
+        int cf = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
+            if (x < tau) {
+                tmpw += expf(x * x / -0.18f);
+                count += 1.0f;
+            }
+            cf += (x > 1.2f);
+        }
+        float p = 0.f;
+        if (count > 2.0f && cf < 3)
+            p = tmpw / count;
+        else if (cf < 3)
+            p = expf(tau * tau / -0.32f);
+        p = p * pri;
+        s_cdf[v][threadIdx.x] = p;
+        psum += p;
+    }
+    const float inv = 1.0f / psum;
+    float cum = 0.f;
+    for (int v = 0; v < nv; ++v) {
+        cum += s_cdf[v][threadIdx.x] * inv;
+        s_cdf[v][threadIdx.x] = cum;
+    }
+    float rp[15];
+    float u_depth_rand;
+    {
+        uint32_t o[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            rng.block(b, o);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = 4 * b + j;
+                if (s < 15)
+                    rp[s] = u01(o[j]) - FLT_EPSILON;
+                else
+                    u_depth_rand = u01(o[j]);
+            }
+        }
+    }
+    uint32_t done = 0, selmask = 0;
+    uint32_t wnib[4] = {0u, 0u, 0u, 0u};  // 4-bit draw counts per view
+    float wsum = 0.f;
+    for (int v = 0; v < nv; ++v) {
+        const float cd = s_cdf[v][threadIdx.x];
+        uint32_t hit = 0;
+#pragma unroll
+        for (int d = 0; d < 15; ++d) hit |= (uint32_t)(cd > rp[d]) << d;
+        const uint32_t fresh = hit & ~done;
+        done |= fresh;
+        const uint32_t cnt = __popc(fresh);
+        if (cnt) {
+            selmask |= 1u << v;
+            const uint32_t sh = 4u * (uint32_t)(v & 7);
+            const uint32_t add = cnt << sh;
+            if ((v >> 3) == 0) wnib[0] |= add;
+            else if ((v >> 3) == 1) wnib[1] |= add;
+            else if ((v >> 3) == 2) wnib[2] |= add;
+            else wnib[3] |= add;
+            wsum += (float)cnt;
+        }
+    }
+    auto weight_of = [&](int v) -> float {
+        const uint32_t wd = (v >> 3) == 0 ? wnib[0] : (v >> 3) == 1 ? wnib[1] : (v >> 3) == 2 ? wnib[2] : wnib[3];
+        return (float)((wd >> (4u * (uint32_t)(v & 7))) & 15u);
+    };
+
+    // ---- candidate final costs (src 983-1009)
+    float fc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const bool fk = (flags >> k) & 1u;
+        float4 cp = make_float4(0.f, 0.f, 0.f, 1.f);
+        if (fk && P.geom) cp = cplane[(size_t)k * NA + aidx];
+        float s = 0.f;
+        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+            const int v = __ffs(mm) - 1;
+            const float wv = weight_of(v);
+            const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
+            if (P.geom && fk)
+                s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cp, fpx, fpy));
+            else
+                s += wv * x;
+        }
+        fc[k] = s / wsum;
+    }
+    const int kmin = argmin_last8(fc);
+
+    Hoist<NT> hs;
+    hoist_ref<NT>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+
+    // Weighted cost of a hypothesis over the selected views (+ geometric term).
+    auto wcost = [&](float4 pl) -> float {
+        float m[3];
+        plane_m(rc, pl, m);
+        float s = 0.f;
+        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+            const int v = __ffs(mm) - 1;
+            const ViewDesc& vd = s_views[v];
+            float H[9];
+            homography(vd.A, vd.b, m, H);
+            const float x = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+            const float wv = weight_of(v);
+            if (P.geom)
+                s += wv * (x + 0.1f * geom_cost(rc, vd, pl, fpx, fpy));
+            else
+                s += wv * x;
+        }
+        return s / wsum;
+    };
+
+    float4 cur = planes[c];
+    const float cnow = wcost(cur);
+    float cost = cnow;
+    uint32_t selc = sel[c];
+    float depth_now = depth_from_plane(rc, cur, fpx, fpy);
+    const float4 cur0 = cur;
+
+    const bool masked = P.prior && P.plane_masks[c] > 0u;
+    float4 pp = make_float4(0.f, 0.f, 0.f, 0.f);
+    float dprior = 0.f;
+    if (P.prior) {
+        pp = P.prior_planes[c];
+        dprior = depth_from_plane(rc, pp, fpx, fpy);
+    }
+    float restricted = 0.f;
+    if (P.prior && masked) {
+        float rs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            rs[k] = 0.f;
+            if ((flags >> k) & 1u) {
+                const float4 cp = cplane[(size_t)k * NA + aidx];
+                rs[k] = prior_score(P, fc[k], depth_from_plane(rc, cp, fpx, fpy), dprior, pp, cp);
+            }
+        }
+        const int kmax = argmax_last8(rs);
+        const float rnow = prior_score(P, cnow, depth_from_plane(rc, cur0, fpx, fpy), dprior, pp, cur0);
+        if ((flags >> kmax) & 1u) {
+            const float4 cp = cplane[(size_t)kmax * NA + aidx];
+            const float db = depth_from_plane(rc, cp, fpx, fpy);
+            if (db >= P.dmin && db <= P.dmax && rs[kmax] > rnow) {
+                depth_now = db;
+                cur = cp;
+                cost = fc[kmax];
+                restricted = rs[kmax];
+                selc = selmask;
+            }
+        }
+    } else if ((flags >> kmin) & 1u) {
+        const float4 cp = cplane[(size_t)kmin * NA + aidx];
+        const float db = depth_from_plane(rc, cp, fpx, fpy);
+        if (db >= P.dmin && db <= P.dmax && fc[kmin] < cnow) {
+            depth_now = db;
+            cur = cp;
+            cost = fc[kmin];
+            if (!P.prior) selc = selmask;  // the prior path leaves it (lib 1306-1318)
+        }
+    }
+
+    // ---- refinement (src 595-676, lib 684-811)
+    const float pert = 0.02f;
+    float drand;
+    float4 nrand;
+    uint32_t o4[4], o5[4];
+    rng.block(4, o4);  // slots 16..19
+    if (P.prior && masked) {
+        const float dsig = (P.dmax - P.dmin) / 64.0f;
+        drand = u_depth_rand * 6.0f * dsig + (dprior - 3.0f * dsig);
+        rng.block(5, o5);  // slots 20..23
+        nrand = perturbed_normal(rc, fpx, fpy, pp, u01(o5[0]), u01(o5[1]), u01(o5[2]),
+                                 kPiF * (5.0f / 180.0f));
+    } else {
+        drand = u_depth_rand * (P.dmax - P.dmin) + P.dmin;
+        nrand = random_normal(rc, fpx, fpy, rng, RNG_NORMAL, depth_now);
+    }
+    const float dlo = (1.0f - pert) * depth_now;
+    const float dhi = (1.0f + pert) * depth_now;
+    const float dpert = u01(o4[0]) * (dhi - dlo) + dlo;
+    const float4 ncur = cur;
+    const float4 npert = perturbed_normal(rc, fpx, fpy, ncur, u01(o4[1]), u01(o4[2]), u01(o4[3]),
+                                          pert * kPiF);
+    const float dcur = depth_now;
+#pragma unroll 1
+    for (int i = 0; i < 5; ++i) {
+        const float di = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dcur);
+        float4 tp = (i == 0 || i == 4) ? ncur : (i == 3 ? npert : nrand);
+        tp.w = dist_to_origin(rc, fpx, fpy, di, tp);
+        const float tc = wcost(tp);
+        const float db = depth_from_plane(rc, tp, fpx, fpy);
+        if (P.prior && masked) {
+            const float sc = prior_score(P, tc, di, dprior, pp, tp);
+            if (db >= P.dmin && db <= P.dmax && sc > restricted) {
+                depth_now = db;
+                cur = tp;
+                cost = tc;
+                restricted = sc;
+            }
+        } else if (db >= P.dmin && db <= P.dmax && tc < cost) {
+            depth_now = db;
+            cur = tp;
+            cost = tc;
+        }
+    }
+    planes[c] = cur;
+    costs[c] = cost;
+    sel[c] = selc;
+}
+
+// ------------------------------------------------------ finalisation ---
+// GetDepthandNormal (lib 1655-1674): .w <- depth, normal camera -> world.
+__global__ void pm_depth_normal(PMConst P, float4* __restrict__ planes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.h * P.w) return;
+    const int py = i / P.w, px = i - py * P.w;
+    const RefCam& rc = P.rc;
+    float4 p = planes[i];
+    p.w = depth_from_plane(rc, p, (float)px, (float)py);
+    const float x = rc.R[0] * p.x + rc.R[3] * p.y + rc.R[6] * p.z;
+    const float y = rc.R[1] * p.x + rc.R[4] * p.y + rc.R[7] * p.z;
+    const float z = rc.R[2] * p.x + rc.R[5] * p.y + rc.R[8] * p.z;
+    planes[i] = make_float4(x, y, z, p.w);
+}
+
+// CheckerboardFilter (src 1165-1281): median of the depth at the pixel and
+// at up to 20 fixed opposite-colour offsets, unless the cost is < 0.001.
+// The median is taken by rank counting over the (at most 21) values.
+__global__ __launch_bounds__(kBlock) void pm_filter(PMConst P, int color,
+                                                    float4* __restrict__ planes,
+                                                    const float* __restrict__ costs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wh = P.wh;
+    if (i >= P.h * wh) return;
+    const int py = i / wh;
+    const int px = 2 * (i - py * wh) + ((py + color) & 1);
+    const int w = P.w, h = P.h;
+    if (px >= w) return;
+    const int c = py * w + px;
+    if (costs[c] < 0.001f) return;
+    // g[0] is the pixel itself, g[1..20] the reference's offsets in order;
+    // gv marks the ones inside the image.
+    const int up = c - w, dn = c + w, lf = c - 1, rt = c + 1;
+    const int q[20] = {up,         c - 3 * w,  c - 5 * w,  dn,         c + 3 * w,
+                       c + 5 * w,  lf,         c - 3,      c - 5,      rt,
+                       c + 3,      c + 5,      up + 2,     dn + 2,     up - 2,
+                       dn - 2,     lf - 2 * w, rt - 2 * w, lf + 2 * w, rt + 2 * w};
+    const bool u[20] = {py > 0,
+                        py > 2,
+                        py > 4,
+                        py < h - 1,
+                        py < h - 3,
+                        py < h - 5,
+                        px > 0,
+                        px > 2,
+                        px > 4,
+                        px < w - 1,
+                        px < w - 3,
+                        px < w - 5,
+                        py > 0 && px < w - 2,
+                        py < h - 1 && px < w - 2,
+                        py > 0 && px > 1,
+                        py < h - 1 && px > 1,
+                        px > 0 && py > 2,
+                        px < w - 1 && py > 2,
+                        px > 0 && py < h - 2,
+                        px < w - 1 && py < h - 2};
+    float g[21];
+    bool gv[21];
+    g[0] = planes[c].w;
+    gv[0] = true;
+#pragma unroll
+    for (int k = 0; k < 20; ++k) {
+        gv[k + 1] = u[k];
+        g[k + 1] = u[k] ? planes[q[k]].w : 0.f;
+    }
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 21; ++k) n += gv[k];
+    // value of rank r: the element e with #(< e) <= r < #(<= e)
+    const int m = n / 2;
+    float lo = 0.f, hi = 0.f;
+#pragma unroll
+    for (int k = 0; k < 21; ++k) {
+        if (!gv[k]) continue;
+        int less = 0, leq = 0;
+#pragma unroll
+        for (int j = 0; j < 21; ++j) {
+            if (!gv[j]) continue;
+            less += g[j] < g[k];
+            leq += g[j] <= g[k];
+        }
+        if (less <= m && m < leq) hi = g[k];
+        if (less <= m - 1 && m - 1 < leq) lo = g[k];
+    }
+    planes[c].w = (n % 2 == 0) ? (lo + hi) / 2.0f : hi;
+}
+
+// export_plane_hypotheses_cost (src 1284-1313).
+__global__ void pm_export(int hw, const float4* __restrict__ planes, const float* __restrict__ costs,
+                          float* __restrict__ depth, float* __restrict__ normal,
+                          float* __restrict__ cost_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hw) return;
+    const float4 p = planes[i];
+    if (depth) depth[i] = p.w;
+    if (normal) {
+        normal[3 * i + 0] = p.x;
+        normal[3 * i + 1] = p.y;
+        normal[3 * i + 2] = p.z;
+    }
+    if (cost_out) cost_out[i] = costs[i];
+}
+
+// Parity hook: V NCCs for a list of (pixel, plane) pairs.
+template <int NT>
+__global__ __launch_bounds__(256) void pm_cost_batch(PMConst P, int n, const int32_t* __restrict__ xy,
+                              const float4* __restrict__ planes, float* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int px = xy[2 * q], py = xy[2 * q + 1];
+    if (px < 0 || px >= P.w || py < 0 || py >= P.h) {
+        for (int v = 0; v < P.nv; ++v) out[(size_t)q * P.nv + v] = __builtin_nanf("");
+        return;
+    }
+    Hoist<NT> hs;
+    hoist_ref<NT>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+    float m[3];
+    plane_m(P.rc, planes[q], m);
+    for (int v = 0; v < P.nv; ++v) {
+        const ViewDesc& vd = P.views[v];
+        float H[9];
+        homography(vd.A, vd.b, m, H);
+        out[(size_t)q * P.nv + v] = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+    }
+}
+
+// ================================================================ host ===
+namespace {
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct PMLayout {
+    size_t views, quads, sel, ca, cplane, cflag, total;
+};
+
+PMLayout pm_layout(int h, int w, int num_images, bool sweep) {
+    const int nv = num_images - 1;
+    const size_t hw = size_t(h) * w;
+    const size_t na = size_t(h) * ((w + 1) / 2);
+    PMLayout L;
+    size_t o = 0;
+    L.views = o;
+    o = align_up(o + sizeof(ViewDesc) * nv);
+    L.quads = o;
+    o = align_up(o + sizeof(float4) * hw * nv);
+    L.sel = o;
+    if (sweep) {
+        o = align_up(o + sizeof(uint32_t) * hw);
+        L.ca = o;
+        o = align_up(o + sizeof(float) * 8 * nv * na);
+        L.cplane = o;
+        o = align_up(o + sizeof(float4) * 8 * na);
+        L.cflag = o;
+        o = align_up(o + na);
+    } else {
+        L.ca = L.cplane = L.cflag = o;
+    }
+    L.total = o;
+    return L;
+}
+
+int taps_per_axis(const gmvs_params& p, int* radius) {
+    *radius = p.patch_size / 2;
+    if (p.radius_increment <= 0) return -1;
+    int n = 0;
+    for (int i = -*radius; i < *radius + 1; i += p.radius_increment) ++n;
+    return n;
+}
+
+int check_cameras(int num_images, const gmvs_camera* cams) {
+    GMVS_ARG_CHECK(cams, "cameras is NULL");
+    GMVS_ARG_CHECK(num_images >= 2 && num_images <= kMaxViews + 1,
+                   "num_images must be in [2, %d] (got %d)", kMaxViews + 1, num_images);
+    const int h = cams[0].height, w = cams[0].width;
+    GMVS_ARG_CHECK(h > 0 && w > 0, "reference image size must be positive (got %dx%d)", w, h);
+    GMVS_ARG_CHECK((long long)h * w < (1LL << 31) / 16, "image too large (%dx%d)", w, h);
+    for (int i = 1; i < num_images; ++i)
+        GMVS_ARG_CHECK(cams[i].height == h && cams[i].width == w,
+                       "all images of a problem must share the reference size (view %d is %dx%d, "
+                       "reference %dx%d)", i, cams[i].width, cams[i].height, w, h);
+    return GMVS_OK;
+}
+
+// Composes the per-view homography factors (FP64, one tiny launch per view)
+// and packs the source images.
+int setup_views(int num_images, const gmvs_camera* cams, const float* const* images,
+                const float* const* depths, char* ws, const PMLayout& L, hipStream_t st) {
+    const int h = cams[0].height, w = cams[0].width;
+    const int nv = num_images - 1;
+    const size_t hw = size_t(h) * w;
+    float4* quads = reinterpret_cast<float4*>(ws + L.quads);
+    ViewDesc* views = reinterpret_cast<ViewDesc*>(ws + L.views);
+    const int blocks = int((hw + 255) / 256);
+    for (int v = 0; v < nv; ++v) {
+        hipLaunchKernelGGL(pm_setup_view, dim3(1), dim3(64), 0, st, cams[0], cams[v + 1],
+                           quads + hw * v, depths ? depths[v + 1] : nullptr, views + v);
+        GMVS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(pm_pack_quads, dim3(blocks), dim3(256), 0, st, images[v + 1],
+                           quads + hw * v, h, w);
+        GMVS_LAUNCH_CHECK();
+    }
+    return GMVS_OK;
+}
+
+PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, const ViewDesc* views,
+                   const gmvs_params& prm, int radius) {
+    PMConst P;
+    std::memset(&P, 0, sizeof(P));
+    const gmvs_camera& r = cams[0];
+    for (int i = 0; i < 9; ++i) {
+        P.rc.K[i] = r.K[i];
+        P.rc.R[i] = r.R[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        P.rc.t[i] = r.t[i];
+        P.rc.C[i] = -(r.R[i] * r.t[0] + r.R[3 + i] * r.t[1] + r.R[6 + i] * r.t[2]);
+    }
+    P.rc.h = r.height;
+    P.rc.w = r.width;
+    P.ref = ref;
+    P.views = views;
+    P.nv = num_images - 1;
+    P.h = r.height;
+    P.w = r.width;
+    P.wh = (r.width + 1) / 2;
+    P.radius = radius;
+    P.inc = prm.radius_increment;
+    P.sigma_s = prm.sigma_spatial;
+    P.sigma_c = prm.sigma_color;
+    P.dmin = prm.depth_min;
+    P.dmax = prm.depth_max;
+    P.top_k = prm.top_k;
+    return P;
+}
+
+}  // namespace
+
+}  // namespace gmvs
+
+using namespace gmvs;
+
+extern "C" size_t gmvs_pm_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
+                                          int32_t precision) {
+    if (height <= 0 || width <= 0 || num_images < 2 || precision != GMVS_PREC_FP32) return 0;
+    return pm_layout(height, width, num_images, true).total;
+}
+
+extern "C" size_t gmvs_cost_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
+                                            int32_t precision) {
+    if (height <= 0 || width <= 0 || num_images < 2 || precision != GMVS_PREC_FP32) return 0;
+    return pm_layout(height, width, num_images, false).total;
+}
+
+extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t workspace_bytes,
+                           void* stream) {
+    clear_error();
+    GMVS_ARG_CHECK(args, "args is NULL");
+    const gmvs_pm_args& A = *args;
+    if (int rc = check_cameras(A.num_images, A.cameras)) return rc;
+    GMVS_ARG_CHECK(A.images, "images is NULL");
+    for (int i = 0; i < A.num_images; ++i) GMVS_ARG_CHECK(A.images[i], "images[%d] is NULL", i);
+    GMVS_ARG_CHECK(A.planes && A.costs, "planes/costs state buffers are required");
+    GMVS_ARG_CHECK(A.precision == GMVS_PREC_FP32, "unsupported precision %d", A.precision);
+    GMVS_ARG_CHECK(A.init_mode >= 0 && A.init_mode <= 2, "bad init_mode %d", A.init_mode);
+    GMVS_ARG_CHECK(A.params.max_iterations >= 0, "max_iterations must be >= 0");
+    GMVS_ARG_CHECK(A.params.top_k >= 0, "top_k must be >= 0");
+    if (A.init_mode == GMVS_INIT_DEPTH_NORMAL)
+        GMVS_ARG_CHECK(A.depths && A.depths[0] && A.ref_normal,
+                       "INIT_DEPTH_NORMAL needs depths[0] and ref_normal");
+    if (A.geom) {
+        GMVS_ARG_CHECK(A.depths, "the geometric term needs source depth maps");
+        for (int i = 1; i < A.num_images; ++i) GMVS_ARG_CHECK(A.depths[i], "depths[%d] is NULL", i);
+    }
+    if (A.params.planar_prior)
+        GMVS_ARG_CHECK(A.prior_planes && A.plane_masks, "planar_prior needs prior planes and masks");
+    int radius = 0;
+    const int nt = taps_per_axis(A.params, &radius);
+    if (nt != 6) {
+        set_error("patch geometry (patch_size=%d, radius_increment=%d) gives %d taps per axis; "
+                  "this build is specialised for 6 (the reference defaults 11/2)",
+                  A.params.patch_size, A.params.radius_increment, nt);
+        return GMVS_ERR_UNSUPPORTED;
+    }
+    const int h = A.cameras[0].height, w = A.cameras[0].width;
+    const PMLayout L = pm_layout(h, w, A.num_images, true);
+    GMVS_ARG_CHECK(workspace, "workspace is NULL");
+    if (workspace_bytes < L.total) {
+        set_error("workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
+        return GMVS_ERR_NOSPACE;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char* ws = reinterpret_cast<char*>(workspace);
+    if (int rc = setup_views(A.num_images, A.cameras, A.images, A.depths, ws, L, st)) return rc;
+
+    PMConst P = make_const(A.num_images, A.cameras, A.images[0],
+                           reinterpret_cast<const ViewDesc*>(ws + L.views), A.params, radius);
+    P.init_depth = A.depths ? A.depths[0] : nullptr;
+    P.init_normal = A.ref_normal;
+    P.prior_planes = reinterpret_cast<const float4*>(A.prior_planes);
+    P.plane_masks = A.plane_masks;
+    P.seed = A.seed;
+    P.view = A.rng_view;
+    P.init_mode = A.init_mode;
+    P.geom = A.geom ? 1 : 0;
+    P.prior = A.params.planar_prior ? 1 : 0;
+
+    float4* planes = reinterpret_cast<float4*>(A.planes);
+    uint32_t* sel = reinterpret_cast<uint32_t*>(ws + L.sel);
+    float* ca = reinterpret_cast<float*>(ws + L.ca);
+    float4* cplane = reinterpret_cast<float4*>(ws + L.cplane);
+    uint8_t* cflag = reinterpret_cast<uint8_t*>(ws + L.cflag);
+    const int na = h * P.wh;
+
+    const int tiles_full = ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
+    hipLaunchKernelGGL(pm_init<6>, dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
+                       A.costs, sel);
+    GMVS_LAUNCH_CHECK();
+    const int tiles_half = ((P.wh + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
+    const dim3 grid_half(grid_for_tiles(tiles_half));
+    for (int it = 0; it < A.params.max_iterations; ++it) {
+        for (int color = 0; color < 2; ++color) {
+            hipLaunchKernelGGL(pm_candidates<6>, grid_half, dim3(kBlock), 0, st, P, color, planes,
+                               A.costs, ca, cplane, cflag, na);
+            GMVS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(pm_select_refine<6>, grid_half, dim3(kBlock), 0, st, P, color, it,
+                               planes, A.costs, sel, ca, cplane, cflag, na);
+            GMVS_LAUNCH_CHECK();
+        }
+    }
+    const int hw = h * w;
+    hipLaunchKernelGGL(pm_depth_normal, dim3((hw + 255) / 256), dim3(256), 0, st, P, planes);
+    GMVS_LAUNCH_CHECK();
+    if (!A.skip_filter) {
+        for (int color = 0; color < 2; ++color) {
+            hipLaunchKernelGGL(pm_filter, dim3((na + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P,
+                               color, planes, A.costs);
+            GMVS_LAUNCH_CHECK();
+        }
+    }
+    if (A.out_depth || A.out_normal || A.out_cost) {
+        hipLaunchKernelGGL(pm_export, dim3((hw + 255) / 256), dim3(256), 0, st, hw, planes, A.costs,
+                           A.out_depth, A.out_normal, A.out_cost);
+        GMVS_LAUNCH_CHECK();
+    }
+    return GMVS_OK;
+}
+
+extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
+                               const float* const* images, const gmvs_params* params,
+                               int32_t precision, int32_t n, const int32_t* xy,
+                               const float* planes, float* costs, void* workspace,
+                               size_t workspace_bytes, void* stream) {
+    clear_error();
+    if (int rc = check_cameras(num_images, cameras)) return rc;
+    GMVS_ARG_CHECK(images && params, "images/params is NULL");
+    for (int i = 0; i < num_images; ++i) GMVS_ARG_CHECK(images[i], "images[%d] is NULL", i);
+    GMVS_ARG_CHECK(precision == GMVS_PREC_FP32, "unsupported precision %d", precision);
+    GMVS_ARG_CHECK(n >= 0, "n must be >= 0");
+    if (n == 0) return GMVS_OK;
+    GMVS_ARG_CHECK(xy && planes && costs, "xy/planes/costs is NULL");
+    int radius = 0;
+    const int nt = taps_per_axis(*params, &radius);
+    if (nt != 6) {
+        set_error("patch geometry gives %d taps per axis; this build is specialised for 6", nt);
+        return GMVS_ERR_UNSUPPORTED;
+    }
+    const int h = cameras[0].height, w = cameras[0].width;
+    const PMLayout L = pm_layout(h, w, num_images, false);
+    GMVS_ARG_CHECK(workspace, "workspace is NULL");
+    if (workspace_bytes < L.total) {
+        set_error("workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
+        return GMVS_ERR_NOSPACE;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char* ws = reinterpret_cast<char*>(workspace);
+    if (int rc = setup_views(num_images, cameras, images, nullptr, ws, L, st)) return rc;
+    PMConst P = make_const(num_images, cameras, images[0],
+                           reinterpret_cast<const ViewDesc*>(ws + L.views), *params, radius);
+    hipLaunchKernelGGL(pm_cost_batch<6>, dim3((n + 255) / 256), dim3(256), 0, st, P, n, xy,
+                       reinterpret_cast<const float4*>(planes), costs);
+    GMVS_LAUNCH_CHECK();
+    return GMVS_OK;
+}

@@ -1,0 +1,190 @@
+"""Drop-in for the reference's ``gmvs.lib.build.planar_initer`` module
+(gmvs/lib/src/bindings.cpp:12-67): ``PlanarInit``, ``PatchMatchParams``,
+``Problem``, ``Camera``, ``load_camera``, ``load_problem``.
+
+PlanarInit runs the first stage on one reference view: a plain PatchMatch
+pass, then (after ``triangulation`` + ``cuda_planar_prior_init``) the
+planar-prior pass. Images live in plain device buffers (no textures); the
+hypothesis state stays on the device between the two passes.
+``fusion_planar_mask``, ``filter_by_var_map`` and ``sliding_window_normal``
+are mesh/segmentation utilities outside this hot path and raise.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from ... import _native as N
+from ... import io as gio
+from ...src import (Camera, PatchMatchParams, Problem, _Workspace, _device, load_camera,
+                    load_problem, run_pm)
+
+__all__ = ["PlanarInit", "PatchMatchParams", "Problem", "Camera", "load_camera", "load_problem",
+           "fusion_planar_mask", "filter_by_var_map", "sliding_window_normal", "build_prior"]
+
+
+def build_prior(ref_cam: N.CCamera, depth: np.ndarray, cost: np.ndarray,
+                labels: Optional[np.ndarray], depth_min: float, depth_max: float,
+                superpixel_filter: bool = False) -> Tuple[np.ndarray, np.ndarray, int]:
+    """Host planar-prior construction (gmvs_prior_build). Returns
+    (prior_planes [H,W,4] f32, plane_masks [H,W] int32, #triangles)."""
+    h, w = ref_cam.height, ref_cam.width
+    depth = np.ascontiguousarray(depth, np.float32).reshape(h, w)
+    cost = np.ascontiguousarray(cost, np.float32).reshape(h, w)
+    lab = None
+    if labels is not None:
+        lab = np.ascontiguousarray(labels, np.int32).reshape(h, w)
+    p = PatchMatchParams()
+    p.depth_min = depth_min
+    p.depth_max = depth_max
+    p.superpixel_filter = superpixel_filter
+    cp = p.to_c()
+    planes = np.zeros((h, w, 4), np.float32)
+    masks = np.zeros((h, w), np.uint32)
+    ntri = ctypes.c_int32(0)
+    N.check("gmvs_prior_build", N.lib.gmvs_prior_build(
+        ctypes.byref(ref_cam), depth.ctypes.data, cost.ctypes.data,
+        lab.ctypes.data if lab is not None else None, ctypes.byref(cp), planes.ctypes.data,
+        masks.ctypes.data, ctypes.byref(ntri)))
+    return planes, masks.astype(np.int32), int(ntri.value)
+
+
+class PlanarInit:
+    """PlanarInit (gmvs/lib/include/planar_initer.cuh:65-133)."""
+
+    def __init__(self, verbose: bool = False):
+        self.verbose = bool(verbose)
+        self.params = PatchMatchParams()
+        self._images: List[np.ndarray] = []
+        self._cameras: List[Camera] = []
+        self._superpixel: Optional[np.ndarray] = None
+        self._dev_images: Optional[List[torch.Tensor]] = None
+        self._planes: Optional[torch.Tensor] = None
+        self._costs: Optional[torch.Tensor] = None
+        self._prior: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
+        self._host_prior: Optional[Tuple[np.ndarray, np.ndarray]] = None
+        self._ws = _Workspace()
+        # Philox key word 1 (the global reference view id; set by the driver)
+        self.rng_view = 0
+
+    # ---- inputs (planar_initer.cu:64-102)
+    def add_image(self, image_path: str) -> None:
+        self._images.append(gio.read_gray(image_path))
+
+    def add_image_array(self, image: np.ndarray) -> None:
+        """Adds an already decoded grey image (float32 [H,W])."""
+        self._images.append(np.ascontiguousarray(image, np.float32))
+
+    def add_superpixel(self, superpixel: np.ndarray) -> None:
+        self._superpixel = np.ascontiguousarray(superpixel).astype(np.int32)
+
+    def get_image(self, index: int) -> np.ndarray:
+        return self._images[index]
+
+    def add_camera(self, K, R, t, height: int, width: int, depth_min: float,
+                   depth_max: float) -> None:
+        self._cameras.append(load_camera(K, R, t, height, width, depth_min, depth_max))
+
+    def get_camera(self, index: int) -> Camera:
+        return self._cameras[index]
+
+    def get_num_images(self) -> int:
+        return len(self._images)
+
+    # ---- device state (planar_initer.cu:119-223)
+    def cuda_space_init(self) -> None:
+        if not self._images or len(self._cameras) < len(self._images):
+            raise RuntimeError("add images and cameras before cuda_space_init")
+        dev = _device()
+        self._dev_images = [torch.from_numpy(im).to(dev).contiguous() for im in self._images]
+        h, w = self._images[0].shape
+        self._planes = torch.zeros(h, w, 4, dtype=torch.float32, device=dev)
+        self._costs = torch.zeros(h, w, dtype=torch.float32, device=dev)
+
+    def cuda_planar_prior_init(self) -> None:
+        if self._host_prior is None:
+            raise RuntimeError("triangulation must run before cuda_planar_prior_init")
+        dev = self._planes.device
+        self._prior = (torch.from_numpy(self._host_prior[0]).to(dev),
+                       torch.from_numpy(self._host_prior[1]).to(dev))
+
+    # ---- PatchMatch (lib 1711-1810)
+    def run_patch_match(self) -> None:
+        if self._dev_images is None:
+            raise RuntimeError("cuda_space_init must run before run_patch_match")
+        n = len(self._dev_images)
+        cp = self.params.to_c()
+        cp.num_images = n
+        prior = bool(self.params.planar_prior)
+        if prior and self._prior is None:
+            raise RuntimeError("planar_prior is set but cuda_planar_prior_init has not run")
+        if self.params.geom_consistency:
+            raise RuntimeError("the planar initialiser has no depth inputs for the geometric "
+                               "term; use gmvs_amd.src.PatchMatcher")
+        run_pm([c._c for c in self._cameras[:n]], self._dev_images, cp,
+               init_mode=N.INIT_PRIOR if prior else N.INIT_RANDOM, geom=False, rng_view=self.rng_view,
+               planes=self._planes, costs=self._costs,
+               prior_planes=self._prior[0] if prior else None,
+               plane_masks=self._prior[1] if prior else None, workspace=self._ws)
+        torch.cuda.synchronize()
+        if self.verbose:
+            for it in range(self.params.max_iterations):
+                print(f"iteration: {it}")
+
+    def jump_patch_match(self) -> None:
+        torch.cuda.synchronize()
+
+    # ---- outputs
+    def export_plane(self, depth: np.ndarray, normal: np.ndarray, cost: np.ndarray) -> None:
+        """Writes into the caller's arrays in place (planar_initer.cu:104-117)."""
+        planes = self._planes.cpu().numpy()
+        costs = self._costs.cpu().numpy()
+        depth[...] = planes[..., 3]
+        normal[...] = planes[..., :3]
+        cost[...] = costs
+
+    def get_cost(self, index: int) -> float:
+        return float(self._costs.reshape(-1)[index].item())
+
+    def triangulation(self, depth: np.ndarray, triangulation_path: str = " ") -> None:
+        """planar_initer.cu:347-420: support points from the last pass's
+        costs, Delaunay, superpixel filter, plane fit and rasterisation."""
+        cam = self._cameras[0]
+        cost = self._costs.cpu().numpy()
+        planes, masks, ntri = build_prior(cam._c, depth, cost, self._superpixel,
+                                          self.params.depth_min, self.params.depth_max,
+                                          bool(self.params.superpixel_filter)
+                                          and self._superpixel is not None)
+        self._host_prior = (planes, masks)
+        self.num_triangles = ntri
+        if triangulation_path.endswith(".png"):
+            if self.verbose:
+                print(f"save triangulation as {triangulation_path}")
+            self._save_prior_png(triangulation_path, masks)
+
+    def _save_prior_png(self, path: str, masks: np.ndarray) -> None:
+        img = self._images[0]
+        edge = np.zeros_like(masks, dtype=bool)
+        edge[:, 1:] |= masks[:, 1:] != masks[:, :-1]
+        edge[1:, :] |= masks[1:, :] != masks[:-1, :]
+        rgb = np.repeat(np.clip(img, 0, 255).astype(np.uint8)[..., None], 3, axis=-1)
+        rgb[edge] = (255, 0, 0)
+        from PIL import Image
+
+        Image.fromarray(rgb).save(path)
+
+
+def fusion_planar_mask(*args, **kwargs):
+    raise NotImplementedError("fusion_planar_mask is a mesh post-processing utility outside "
+                              "the PatchMatch hot path")
+
+
+def filter_by_var_map(*args, **kwargs):
+    raise NotImplementedError("filter_by_var_map is outside the PatchMatch hot path")
+
+
+def sliding_window_normal(*args, **kwargs):
+    raise NotImplementedError("sliding_window_normal is outside the PatchMatch hot path")

@@ -1,0 +1,190 @@
+/*
+ * gmvs_amd -- MI355X (gfx950) PatchMatch multi-view stereo hot path.
+ *
+ * C ABI of libgmvs_amd.so: plain structs, plain pointers and sizes, an int
+ * status (0 = ok) and a per-thread error string. No torch or HIP types
+ * appear in the signatures; `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream). Every device pointer must be on the current
+ * HIP device.
+ *
+ * Each entry point names the reference interface it replaces. Citations are
+ * relative to the gmvs source tree (Gorilla-Lab-SCUT/gmvs).
+ */
+#ifndef GMVS_AMD_H
+#define GMVS_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMVS_MAX_NGB 32 /* MAX_NGB, gmvs/include/data_structure.hpp:10 */
+
+/* Status codes. */
+enum {
+    GMVS_OK = 0,
+    GMVS_ERR_ARG = 1,      /* invalid argument (shape, count, NULL pointer) */
+    GMVS_ERR_HIP = 2,      /* a HIP runtime call failed */
+    GMVS_ERR_NOSPACE = 3,  /* workspace too small */
+    GMVS_ERR_UNSUPPORTED = 4
+};
+
+/* mvs::Camera (gmvs/include/data_structure.hpp:45-53): K, world->camera R, t
+ * (X_c = R X_w + t), image size and depth range. Same 100-byte layout. */
+typedef struct gmvs_camera {
+    float K[9];
+    float R[9];
+    float t[3];
+    int32_t height;
+    int32_t width;
+    float depth_min;
+    float depth_max;
+} gmvs_camera;
+
+/* mvs::Problem (gmvs/include/data_structure.hpp:39-43). */
+typedef struct gmvs_problem {
+    int32_t num_ngb;
+    int32_t ref_image_id;
+    int32_t src_image_ids[GMVS_MAX_NGB];
+} gmvs_problem;
+
+/* PatchMatchParams (gmvs/lib/include/planar_initer.hpp:13-31, which is
+ * mvs::PatchMatchParams of data_structure.hpp:20-37 plus superpixel_filter). */
+typedef struct gmvs_params {
+    int32_t max_iterations;
+    int32_t patch_size;
+    int32_t num_images;
+    int32_t max_image_size;
+    int32_t radius_increment;
+    float sigma_spatial;
+    float sigma_color;
+    int32_t top_k;
+    float baseline;
+    float depth_min;
+    float depth_max;
+    float disparity_min;
+    float disparity_max;
+    bool geom_consistency;
+    bool multi_geometry;
+    bool planar_prior;
+    bool superpixel_filter;
+} gmvs_params;
+
+/* Initialisation of the plane hypotheses (A23). */
+enum {
+    GMVS_INIT_RANDOM = 0,       /* RandomInitialization random branch, lib 1553-1575 */
+    GMVS_INIT_DEPTH_NORMAL = 1, /* initialization, src 1046-1082 */
+    GMVS_INIT_PRIOR = 2         /* RandomInitialization prior branch, lib 1513-1551 */
+};
+
+/* Arithmetic of the image samples and patch products. */
+enum {
+    GMVS_PREC_FP32 = 0 /* plain FP32 path */
+};
+
+/* One PatchMatch pass over one reference view. */
+typedef struct gmvs_pm_args {
+    int32_t num_images;           /* reference + sources, 2..32 */
+    const gmvs_camera* cameras;   /* host [num_images], reference first */
+    const float* const* images;   /* host array of num_images device ptrs, [H*W] f32 */
+    const float* const* depths;   /* host array of num_images device ptrs, [H*W] f32, or NULL:
+                                     [0] = input depth (INIT_DEPTH_NORMAL), [1..] = geom maps */
+    const float* ref_normal;      /* device [H*W*3] world normals (INIT_DEPTH_NORMAL) or NULL */
+    const float* prior_planes;    /* device [H*W*4] (n, d) per pixel, or NULL */
+    const uint32_t* plane_masks;  /* device [H*W] triangle id + 1, 0 = no prior, or NULL */
+    float* planes;                /* device [H*W*4] state: in for INIT_PRIOR, out always
+                                     ((n_world, depth) after the pass) */
+    float* costs;                 /* device [H*W] state: in for INIT_PRIOR, out always */
+    gmvs_params params;           /* depth_min/depth_max are used as given */
+    int32_t init_mode;            /* GMVS_INIT_* */
+    int32_t geom;                 /* 1: add 0.1 * geometric-consistency cost (src 657) */
+    uint32_t seed;                /* Philox key word 0 (reference seed 3407) */
+    uint32_t rng_view;            /* Philox key word 1 (global reference view id) */
+    int32_t precision;            /* GMVS_PREC_* */
+    int32_t skip_filter;          /* 1: skip CheckerboardFilter */
+    float* out_depth;             /* device [H*W] or NULL */
+    float* out_normal;            /* device [H*W*3] or NULL */
+    float* out_cost;              /* device [H*W] or NULL */
+} gmvs_pm_args;
+
+/* Last error message of the calling thread ("" if none). */
+const char* gmvs_last_error(void);
+
+/* Library version string. */
+const char* gmvs_version(void);
+
+/* mvs::load_camera (gmvs/src/data_structure.cpp:4-29;
+ * gmvs/lib/src/planar_initer.cpp:3-28). */
+int gmvs_load_camera(const float K[9], const float R[9], const float t[3], int32_t height,
+                     int32_t width, float depth_min, float depth_max, gmvs_camera* out);
+
+/* mvs::load_problem (gmvs/src/data_structure.cpp:31-39). More than
+ * GMVS_MAX_NGB sources is an error (the reference asserts). */
+int gmvs_load_problem(int32_t ref_id, const int32_t* src_ids, int32_t num_src, gmvs_problem* out);
+
+/* Device workspace needed by gmvs_pm_run for one reference view. */
+size_t gmvs_pm_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
+                               int32_t precision);
+
+/* One PatchMatch pass: init, max_iterations red/black sweeps, depth/normal,
+ * median filter and export. Replaces PatchMatcher::run_patch_match
+ * (gmvs/src/patch_match.cu:1353-1519) and PlanarInit::run_patch_match
+ * (gmvs/lib/src/patch_match_kernel.cu:1711-1810). Asynchronous on `stream`. */
+int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Parity hook: V bilateral NCCs for n (pixel, plane) pairs -- the batched form
+ * of ComputeMultiViewCostVector (gmvs/lib/src/patch_match_kernel.cu:571-588,
+ * gmvs/src/patch_match.cu:503-522). xy: device [n*2] int32, planes: device
+ * [n*4] f32 (camera-frame (n, d)), costs: device [n*(num_images-1)] f32. */
+size_t gmvs_cost_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
+                                 int32_t precision);
+int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras, const float* const* images,
+                    const gmvs_params* params, int32_t precision, int32_t n, const int32_t* xy,
+                    const float* planes, float* costs, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
+/* Fusion of one reference view -- Fuser::run_fusion + batch_fusion_kernel
+ * (gmvs/src/fusion.cu:61-182, 211-275). depths/normals: host arrays of
+ * num_images device pointers (reference first). masks: device
+ * [num_mask_views*H*W] bytes (bool), indexed by ref_id and src_ids (global
+ * view ids, each < num_mask_views), updated in place.
+ * out_depth [H*W] and out_normal [H*W*3] are written (zeroed first).
+ * Thresholds: reference values are 2 px, 0.01 and cos(0.174533 rad). */
+size_t gmvs_fuse_workspace_bytes(int32_t height, int32_t width);
+int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras, const float* const* depths,
+                   const float* const* normals, uint8_t* masks, int32_t num_mask_views,
+                   int32_t ref_id,
+                   const int32_t* src_ids, int32_t geom_consistent, float reproj_px,
+                   float rel_depth, float cos_angle, float* out_depth, float* out_normal,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* Planar prior construction on the host -- PlanarInit::triangulation and
+ * cuda_planar_prior_init (gmvs/lib/src/planar_initer.cu:195-420, 686-745):
+ * support points from the pass-1 cost, Delaunay triangulation, optional
+ * superpixel filter (int32 labels), 3-point plane fit, rasterisation.
+ * depth/cost: host [H*W]; labels: host [H*W] int32 or NULL;
+ * out prior_planes: host [H*W*4]; out plane_masks: host [H*W] (0 = none).
+ * out_num_triangles may be NULL. */
+int gmvs_prior_build(const gmvs_camera* ref_camera, const float* depth, const float* cost,
+                     const int32_t* labels, const gmvs_params* params, float* prior_planes,
+                     uint32_t* plane_masks, int32_t* out_num_triangles);
+
+/* Delaunay triangulation used by gmvs_prior_build, exported for its tests;
+ * replaces the cv::Subdiv2D insert / getTriangleList calls of
+ * PlanarInit::triangulation (gmvs/lib/src/planar_initer.cu:354-361).
+ * xy: host [n*2] int32 in [0, 65536); tri_out: host [max_tri*3] vertex
+ * indices (counter-clockwise) or NULL; *num_tri receives the count. */
+int gmvs_delaunay(const int32_t* xy, int32_t n, int32_t* tri_out, int32_t max_tri,
+                  int32_t* num_tri);
+
+/* Philox4x32-10 block (host), exported for known-answer tests. */
+void gmvs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMVS_AMD_H */

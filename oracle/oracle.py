@@ -1,0 +1,244 @@
+"""ctypes wrapper of liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker (or the timed CPU baseline). The product
+(gmvs_amd/) never imports it.
+
+    pm_run(...)        one PatchMatch pass (precision 64 = FP64 oracle,
+                       32 = the OpenMP FP32 CPU baseline)
+    cost_batch(...)    FP64 bilateral NCC of (pixel, plane) queries
+    geom_batch(...)    FP64 geometric-consistency cost of (pixel, plane) queries
+    fuse_view(...)     sequential FP32 fusion twin (bit-exact reference)
+    philox(...)        the oracle's own Philox4x32-10 block
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build() -> str:
+    res = subprocess.run(["make", "-C", HERE, "-j4"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + res.stdout + res.stderr)
+    return LIB_PATH
+
+
+class OCamera(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_float * 9), ("R", ctypes.c_float * 9), ("t", ctypes.c_float * 3),
+                ("height", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float)]
+
+
+class OParams(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int32), ("patch_size", ctypes.c_int32),
+                ("num_images", ctypes.c_int32), ("max_image_size", ctypes.c_int32),
+                ("radius_increment", ctypes.c_int32), ("sigma_spatial", ctypes.c_float),
+                ("sigma_color", ctypes.c_float), ("top_k", ctypes.c_int32),
+                ("baseline", ctypes.c_float), ("depth_min", ctypes.c_float),
+                ("depth_max", ctypes.c_float), ("disparity_min", ctypes.c_float),
+                ("disparity_max", ctypes.c_float), ("geom_consistency", ctypes.c_bool),
+                ("multi_geometry", ctypes.c_bool), ("planar_prior", ctypes.c_bool),
+                ("superpixel_filter", ctypes.c_bool)]
+
+
+class OPMArgs(ctypes.Structure):
+    _fields_ = [("cameras", ctypes.POINTER(OCamera)), ("num_images", ctypes.c_int32),
+                ("images", ctypes.POINTER(ctypes.c_void_p)),
+                ("depths", ctypes.POINTER(ctypes.c_void_p)),
+                ("ref_normal", ctypes.c_void_p), ("prior_planes", ctypes.c_void_p),
+                ("plane_masks", ctypes.c_void_p), ("planes", ctypes.c_void_p),
+                ("costs", ctypes.c_void_p), ("selected_views", ctypes.c_void_p),
+                ("params", OParams), ("init_mode", ctypes.c_int32), ("geom", ctypes.c_int32),
+                ("seed", ctypes.c_uint32), ("rng_view", ctypes.c_uint32),
+                ("threads", ctypes.c_int32), ("skip_filter", ctypes.c_int32),
+                ("out_depth", ctypes.c_void_p), ("out_normal", ctypes.c_void_p),
+                ("out_cost", ctypes.c_void_p)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.oracle_philox4x32_10.argtypes = [vp, vp, vp]
+    lib.oracle_philox4x32_10.restype = None
+    lib.oracle_cost_batch.argtypes = [ctypes.POINTER(OCamera), ctypes.c_int,
+                                      ctypes.POINTER(vp), ctypes.POINTER(OParams), ctypes.c_int,
+                                      vp, vp, vp, vp, vp]
+    lib.oracle_geom_batch.argtypes = [ctypes.POINTER(OCamera), ctypes.c_int, ctypes.POINTER(vp),
+                                      ctypes.c_int, vp, vp, vp]
+    lib.oracle_pm_run.argtypes = [ctypes.c_int, ctypes.POINTER(OPMArgs)]
+    lib.oracle_max_threads.restype = ctypes.c_int
+    lib.oracle_fuse_view.argtypes = [ctypes.POINTER(OCamera), ctypes.c_int, ctypes.POINTER(vp),
+                                     ctypes.POINTER(vp), vp, ctypes.c_int, vp, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
+    lib.oracle_fuse_view.restype = ctypes.c_int64
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def make_camera(K, R, t, height, width, depth_min, depth_max) -> OCamera:
+    c = OCamera()
+    c.K[:] = [float(x) for x in np.asarray(K, np.float64).reshape(9)]
+    c.R[:] = [float(x) for x in np.asarray(R, np.float64).reshape(9)]
+    c.t[:] = [float(x) for x in np.asarray(t, np.float64).reshape(3)]
+    c.height, c.width = int(height), int(width)
+    c.depth_min, c.depth_max = float(depth_min), float(depth_max)
+    return c
+
+
+def make_params(**kw) -> OParams:
+    p = OParams()
+    defaults = dict(max_iterations=3, patch_size=11, num_images=5, max_image_size=3200,
+                    radius_increment=2, sigma_spatial=5.0, sigma_color=3.0, top_k=4,
+                    baseline=0.54, depth_min=0.0, depth_max=1.0, disparity_min=0.0,
+                    disparity_max=1.0, geom_consistency=False, multi_geometry=False,
+                    planar_prior=False, superpixel_filter=False)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+def philox(ctr: Sequence[int], key: Sequence[int]) -> List[int]:
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().oracle_philox4x32_10(_ptr(c), _ptr(k), _ptr(o))
+    return [int(x) for x in o]
+
+
+def _cams(cams: Sequence[OCamera]):
+    arr = (OCamera * len(cams))()
+    for i, c in enumerate(cams):
+        arr[i] = c
+    return arr
+
+
+def _ptrs(arrs: Sequence[Optional[np.ndarray]]):
+    out = (ctypes.c_void_p * len(arrs))()
+    for i, a in enumerate(arrs):
+        out[i] = None if a is None else a.ctypes.data
+    return out
+
+
+def cost_batch(cams: Sequence[OCamera], images: Sequence[np.ndarray], params: OParams,
+               xy: np.ndarray, planes: np.ndarray):
+    """Returns (costs, edge_px, min_var), each [n, V] float64."""
+    imgs = [_f32(a) for a in images]
+    xy = np.ascontiguousarray(xy, np.int32)
+    planes = _f32(planes)
+    n, nv = xy.shape[0], len(imgs) - 1
+    costs = np.zeros((n, nv), np.float64)
+    edge = np.zeros((n, nv), np.float64)
+    minvar = np.zeros((n, nv), np.float64)
+    rc = lib().oracle_cost_batch(_cams(cams), len(imgs), _ptrs(imgs), ctypes.byref(params), n,
+                                 _ptr(xy), _ptr(planes), _ptr(costs), _ptr(edge), _ptr(minvar))
+    assert rc == 0, rc
+    return costs, edge, minvar
+
+
+def geom_batch(cams: Sequence[OCamera], depths: Sequence[np.ndarray], xy: np.ndarray,
+               planes: np.ndarray) -> np.ndarray:
+    ds = [_f32(a) for a in depths]
+    xy = np.ascontiguousarray(xy, np.int32)
+    planes = _f32(planes)
+    out = np.zeros((xy.shape[0], len(ds) - 1), np.float64)
+    rc = lib().oracle_geom_batch(_cams(cams), len(ds), _ptrs(ds), xy.shape[0], _ptr(xy),
+                                 _ptr(planes), _ptr(out))
+    assert rc == 0, rc
+    return out
+
+
+def pm_run(cams: Sequence[OCamera], images: Sequence[np.ndarray], params: OParams, *,
+           precision: int = 64, init_mode: int = 0, geom: bool = False, seed: int = 3407,
+           rng_view: int = 0, depths: Optional[Sequence[np.ndarray]] = None,
+           ref_normal: Optional[np.ndarray] = None, prior_planes: Optional[np.ndarray] = None,
+           plane_masks: Optional[np.ndarray] = None, planes: Optional[np.ndarray] = None,
+           costs: Optional[np.ndarray] = None, threads: int = 0, skip_filter: bool = False):
+    """One PatchMatch pass. Returns dict(depth, normal, cost, planes, selected_views)."""
+    imgs = [_f32(a) for a in images]
+    h, w = imgs[0].shape
+    hw = h * w
+    st_planes = np.zeros((hw, 4), np.float32) if planes is None else _f32(planes).reshape(hw, 4).copy()
+    st_costs = np.zeros(hw, np.float32) if costs is None else _f32(costs).reshape(hw).copy()
+    sel = np.zeros(hw, np.uint32)
+    od = np.zeros((h, w), np.float32)
+    on = np.zeros((h, w, 3), np.float32)
+    oc = np.zeros((h, w), np.float32)
+    ds = None if depths is None else [_f32(a) for a in depths]
+    rn = None if ref_normal is None else _f32(ref_normal)
+    pp = None if prior_planes is None else _f32(prior_planes)
+    pmk = None if plane_masks is None else np.ascontiguousarray(plane_masks, np.uint32)
+    a = OPMArgs()
+    cam_arr = _cams(cams)
+    img_ptrs = _ptrs(imgs)
+    dep_ptrs = _ptrs(ds) if ds is not None else None
+    a.cameras = ctypes.cast(cam_arr, ctypes.POINTER(OCamera))
+    a.num_images = len(imgs)
+    a.images = ctypes.cast(img_ptrs, ctypes.POINTER(ctypes.c_void_p))
+    a.depths = ctypes.cast(dep_ptrs, ctypes.POINTER(ctypes.c_void_p)) if dep_ptrs is not None else None
+    a.ref_normal = _ptr(rn)
+    a.prior_planes = _ptr(pp)
+    a.plane_masks = _ptr(pmk)
+    a.planes = _ptr(st_planes)
+    a.costs = _ptr(st_costs)
+    a.selected_views = _ptr(sel)
+    a.params = params
+    a.params.num_images = len(imgs)
+    a.init_mode = init_mode
+    a.geom = 1 if geom else 0
+    a.seed = seed
+    a.rng_view = rng_view
+    a.threads = threads
+    a.skip_filter = 1 if skip_filter else 0
+    a.out_depth = _ptr(od)
+    a.out_normal = _ptr(on)
+    a.out_cost = _ptr(oc)
+    rc = lib().oracle_pm_run(precision, ctypes.byref(a))
+    assert rc == 0, f"oracle_pm_run returned {rc}"
+    return dict(depth=od, normal=on, cost=oc, planes=st_planes.reshape(h, w, 4),
+                selected_views=sel.reshape(h, w))
+
+
+def fuse_view(cams: Sequence[OCamera], depths: Sequence[np.ndarray],
+              normals: Sequence[np.ndarray], masks: np.ndarray, ref_id: int,
+              src_ids: Sequence[int], geom_consistent: int, reproj_px: float,
+              rel_depth: float, cos_angle: float):
+    """Sequential FP32 fusion of one view; masks ([N,H,W] uint8) is updated in place."""
+    ds = [_f32(a) for a in depths]
+    ns = [_f32(a) for a in normals]
+    h, w = ds[0].shape
+    od = np.zeros((h, w), np.float32)
+    on = np.zeros((h, w, 3), np.float32)
+    sids = np.ascontiguousarray(src_ids, np.int32)
+    assert masks.dtype == np.uint8 and masks.flags.c_contiguous
+    n = lib().oracle_fuse_view(_cams(cams), len(ds), _ptrs(ds), _ptrs(ns), _ptr(masks), ref_id,
+                               _ptr(sids), geom_consistent, reproj_px, rel_depth, cos_angle,
+                               _ptr(od), _ptr(on))
+    return od, on, int(n)

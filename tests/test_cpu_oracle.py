@@ -1,0 +1,179 @@
+"""CPU tests pinning the FP64 oracle by closed forms, invariants and brute force.
+
+Each check is chosen so that a plausible mistake in the restatement (a
+transposed rotation, a wrong sign in the homography, swapped tap axes, an
+unclamped index) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as T
+
+
+def _cam(K, R, t, h, w, dmin=1.0, dmax=10.0):
+    return O.make_camera(K, R, t, h, w, dmin, dmax)
+
+
+def _K(f, w, h):
+    return np.array([[f, 0, w / 2.0], [0, f, h / 2.0], [0, 0, 1.0]])
+
+
+def _texture(h, w, seed=0):
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, 256, (h // 4 + 2, w // 4 + 2)).astype(np.float32)
+    img = np.kron(base, np.ones((4, 4), np.float32))[:h, :w]
+    return img + rng.integers(0, 8, (h, w)).astype(np.float32)
+
+
+def _fronto(depth):
+    return np.array([0.0, 0.0, -1.0, depth], np.float32)  # n = -z, d = depth
+
+
+def test_self_match_is_zero_cost():
+    h, w = 40, 60
+    img = _texture(h, w)
+    K = _K(50.0, w, h)
+    cam = _cam(K, np.eye(3), [0, 0, 0], h, w)
+    xy = np.array([[30, 20], [7, 9], [52, 33]], np.int32)
+    planes = np.stack([_fronto(3.0)] * 3)
+    c, edge, mv = O.cost_batch([cam, cam], [img, img], O.make_params(num_images=2), xy, planes)
+    assert np.all(c < 1e-9)
+
+
+@pytest.mark.parametrize("a,b,expect", [(2.0, 10.0, 0.0), (-1.0, 255.0, 2.0), (0.5, -3.0, 0.0)])
+def test_affine_intensity_invariance(a, b, expect):
+    h, w = 40, 60
+    img = _texture(h, w, 1)
+    K = _K(50.0, w, h)
+    cam = _cam(K, np.eye(3), [0, 0, 0], h, w)
+    xy = np.array([[30, 20], [20, 15]], np.int32)
+    planes = np.stack([_fronto(3.0)] * 2)
+    c, _, _ = O.cost_batch([cam, cam], [img, a * img + b], O.make_params(num_images=2), xy, planes)
+    assert np.allclose(c, expect, atol=1e-9)
+
+
+def test_flat_patch_and_out_of_bounds_cost_two():
+    h, w = 40, 60
+    img = _texture(h, w, 2)
+    flat = np.full((h, w), 77.0, np.float32)
+    K = _K(50.0, w, h)
+    cam = _cam(K, np.eye(3), [0, 0, 0], h, w)
+    xy = np.array([[30, 20]], np.int32)
+    c, _, mv = O.cost_batch([cam, cam], [img, flat], O.make_params(num_images=2), xy,
+                            _fronto(3.0)[None])
+    assert c[0, 0] == 2.0 and mv[0, 0] < 1e-5
+    # a source camera shifted 100 units sideways sees the centre out of bounds
+    cam2 = _cam(K, np.eye(3), [100.0, 0, 0], h, w)
+    c, edge, _ = O.cost_batch([cam, cam2], [img, img], O.make_params(num_images=2), xy,
+                              _fronto(3.0)[None])
+    assert c[0, 0] == 2.0
+
+
+def test_pure_shift_homography_matches_shifted_image():
+    # fronto-parallel plane at depth z, source camera translated by baseline b
+    # along +x: every pixel maps to x - f*b/z. With f*b/z = 4 px the source is
+    # the reference shifted by 4 columns.
+    h, w = 48, 80
+    f, z = 40.0, 5.0
+    b = 4.0 * z / f
+    img = _texture(h, w, 3)
+    src = np.zeros_like(img)
+    src[:, :-4] = img[:, 4:]
+    K = _K(f, w, h)
+    ref_cam = _cam(K, np.eye(3), [0, 0, 0], h, w)
+    src_cam = _cam(K, np.eye(3), [-b, 0, 0], h, w)  # C_s = (b, 0, 0)
+    xy = np.array([[40, 24], [30, 10], [60, 35]], np.int32)
+    c, _, _ = O.cost_batch([ref_cam, src_cam], [img, src], O.make_params(num_images=2), xy,
+                           np.stack([_fronto(z)] * 3))
+    assert np.all(c < 1e-9), c
+    # the wrong direction of the shift must not match
+    src_bad = np.zeros_like(img)
+    src_bad[:, 4:] = img[:, :-4]
+    c_bad, _, _ = O.cost_batch([ref_cam, src_cam], [img, src_bad], O.make_params(num_images=2), xy,
+                               np.stack([_fronto(z)] * 3))
+    assert np.all(c_bad > 0.05)
+
+
+def test_homography_agrees_with_point_projection_on_real_scene():
+    # For a plane through the ground-truth surface point, a source image
+    # rendered from the same plane matches: the ground-truth plane's cost is
+    # far below a random plane's at most interior pixels.
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    rng = np.random.default_rng(0)
+    xy = np.stack([rng.integers(40, 280, 300), rng.integers(30, 210, 300)], 1).astype(np.int32)
+    gt = T.gt_planes(sc, 0, xy)
+    dmin, dmax = T.depth_range(sc, 0)
+    _, rnd = T.random_planes(sc, 0, 300, rng, dmin, dmax)
+    cams = T.oracle_cams(sc, ids)
+    imgs = [sc.images[i].numpy() for i in ids]
+    cg, _, _ = O.cost_batch(cams, imgs, O.make_params(num_images=5), xy, gt)
+    cr, _, _ = O.cost_batch(cams, imgs, O.make_params(num_images=5), xy, rnd)
+    best_g = np.sort(cg, 1)[:, :2].mean(1)
+    best_r = np.sort(cr, 1)[:, :2].mean(1)
+    assert np.median(best_g) < 0.1
+    assert np.median(best_r) > 0.4
+
+
+def test_geom_cost_zero_on_consistent_depth_and_five_on_empty():
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    rng = np.random.default_rng(1)
+    xy = np.stack([rng.integers(60, 260, 200), rng.integers(40, 200, 200)], 1).astype(np.int32)
+    gt = T.gt_planes(sc, 0, xy)
+    depths = [sc.depth[i].numpy() for i in ids]
+    g = O.geom_batch(T.oracle_cams(sc, ids), depths, xy, gt)
+    # visible and away from depth edges: reprojection error ~ 0
+    assert np.median(g) < 0.05
+    zero = [np.zeros_like(d) for d in depths]
+    g0 = O.geom_batch(T.oracle_cams(sc, ids), zero, xy, gt)
+    assert np.all(g0 == 5.0)
+
+
+def test_fp64_oracle_recovers_ground_truth_and_fp32_tracks_it():
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    dmin, dmax = T.depth_range(sc, 0)
+    p = O.make_params(max_iterations=3, depth_min=dmin, depth_max=dmax)
+    cams = T.oracle_cams(sc, ids)
+    imgs = [sc.images[i].numpy() for i in ids]
+    o64 = O.pm_run(cams, imgs, p, precision=64)
+    o32 = O.pm_run(cams, imgs, p, precision=32)
+    gt = sc.depth[0].numpy()
+    assert float((np.abs(o64["depth"] - gt) / gt < 0.01).mean()) > 0.8
+    agree = T.depth_normal_agreement(o32["depth"], o32["normal"], o64["depth"], o64["normal"])
+    assert agree > 0.99
+    # deterministic
+    again = O.pm_run(cams, imgs, p, precision=64)
+    assert np.array_equal(again["depth"], o64["depth"], equal_nan=True)
+
+
+def test_filter_median_matches_numpy_on_constant_cost():
+    # with skip_filter the exported depth is the raw plane depth; the filter
+    # replaces it by the median over the fixed opposite-colour offsets
+    sc = T.scene()
+    ids = [0, 1, 2]
+    dmin, dmax = T.depth_range(sc, 0)
+    p = O.make_params(max_iterations=1, depth_min=dmin, depth_max=dmax)
+    cams = T.oracle_cams(sc, ids)
+    imgs = [sc.images[i].numpy() for i in ids]
+    raw = O.pm_run(cams, imgs, p, precision=64, skip_filter=True)
+    filt = O.pm_run(cams, imgs, p, precision=64)
+    d, cost = raw["depth"], raw["cost"]
+    h, w = d.shape
+    offs = [(0, -1), (0, -3), (0, -5), (0, 1), (0, 3), (0, 5), (-1, 0), (-3, 0), (-5, 0), (1, 0),
+            (3, 0), (5, 0), (2, -1), (2, 1), (-2, -1), (-2, 1), (-1, -2), (1, -2), (-1, 2), (1, 2)]
+    # black pixels ((x + y) even) are filtered first from unfiltered red ones
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        # interior pixels: all 20 offsets present
+        y, x = int(rng.integers(5, h - 5)), int(rng.integers(5, w - 5))
+        if (x + y) % 2 or not (cost[y, x] >= 0.001):
+            continue
+        vals = [d[y, x]] + [d[y + dy, x + dx] for dx, dy in offs]
+        if not np.all(np.isfinite(vals)):
+            continue
+        assert filt["depth"][y, x] == pytest.approx(float(np.median(np.float32(vals))), rel=1e-6)

@@ -1,0 +1,65 @@
+"""GPU fusion vs the sequential FP32 twin: bit-exact masks, outputs, counts."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests import helpers as T
+
+pytestmark = pytest.mark.gpu
+
+
+def noisy_maps(sc, rng, noise=0.01, outliers=0.05):
+    n = sc.depth.shape[0]
+    ds, ns = [], []
+    for i in range(n):
+        d = sc.depth[i].numpy().copy()
+        d *= (1 + noise * rng.standard_normal(d.shape)).astype(np.float32)
+        out = rng.random(d.shape) < outliers
+        d[out] = rng.uniform(1.5, 7.0, out.sum()).astype(np.float32)
+        nn = sc.normal[i].numpy().copy()
+        ds.append(d.astype(np.float32))
+        ns.append(nn.astype(np.float32))
+    return ds, ns
+
+
+@pytest.mark.parametrize("thresholds", [(2.0, 0.01, 0.174533, 3), (1.0, 0.01, math.radians(30), 2)])
+def test_fusion_bit_exact_against_twin(thresholds):
+    from gmvs_amd import src as S
+
+    reproj, rel, ang, gc = thresholds
+    sc = T.scene()
+    nviews = sc.depth.shape[0]
+    rng = np.random.default_rng(7)
+    ds, ns = noisy_maps(sc, rng)
+    cams_c = [S.load_camera(c.K.reshape(-1), c.R.reshape(-1), c.t, c.height, c.width,
+                            c.depth_min, c.depth_max)._c for c in sc.cameras]
+    ocams = T.oracle_cams(sc, range(nviews))
+    dev = torch.device("cuda")
+    dd = [torch.from_numpy(d).to(dev) for d in ds]
+    nd = [torch.from_numpy(n).to(dev) for n in ns]
+    h, w = ds[0].shape
+    masks_g = torch.zeros(nviews, h, w, dtype=torch.bool, device=dev)
+    masks_o = np.zeros((nviews, h, w), np.uint8)
+    cos_t = float(np.float32(math.cos(ang)))
+    total_g = total_o = 0
+    for ref in range(nviews):
+        ids = [ref] + sc.pairs[ref]
+        od = torch.empty(h, w, device=dev)
+        on = torch.empty(h, w, 3, device=dev)
+        S.fuse_view(ids, [cams_c[i] for i in ids], [dd[i] for i in ids], [nd[i] for i in ids],
+                    masks_g, gc, od, on, reproj, rel, ang)
+        torch.cuda.synchronize()
+        od_o, on_o, _ = O.fuse_view([ocams[i] for i in ids], [ds[i] for i in ids],
+                                    [ns[i] for i in ids], masks_o, ref, ids[1:], gc, reproj, rel,
+                                    cos_t)
+        gd, gn = od.cpu().numpy(), on.cpu().numpy()
+        assert np.array_equal(gd.view(np.uint32), od_o.view(np.uint32)), f"depth of view {ref}"
+        assert np.array_equal(gn.view(np.uint32), on_o.view(np.uint32)), f"normal of view {ref}"
+        assert np.array_equal(masks_g.cpu().numpy().astype(np.uint8), masks_o), f"masks after {ref}"
+        total_g += int((gd > 0).sum())
+        total_o += int((od_o > 0).sum())
+    print(f"fused points: GPU {total_g}  twin {total_o}")
+    assert total_g == total_o and total_g > 0

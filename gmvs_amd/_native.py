@@ -106,8 +106,16 @@ EXPORTS = (
     "gmvs_fuse_view",
     "gmvs_prior_build",
     "gmvs_delaunay",
+    "gmvs_profile_enable",
+    "gmvs_profile_collect",
+    "gmvs_profile_reset",
     "gmvs_philox4x32_10",
 )
+
+
+class CKernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64),
+                ("total_ms", ctypes.c_double)]
 
 
 class GmvsError(RuntimeError):
@@ -147,6 +155,8 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_delaunay.argtypes = [vp, i32, vp, i32, ctypes.POINTER(i32)]
     lib.gmvs_philox4x32_10.argtypes = [vp, vp, vp]
     lib.gmvs_philox4x32_10.restype = None
+    lib.gmvs_profile_enable.argtypes = [i32]
+    lib.gmvs_profile_collect.argtypes = [ctypes.POINTER(CKernelStat), i32, ctypes.POINTER(i32)]
     for name in EXPORTS:
         getattr(lib, name)
     return lib
@@ -161,6 +171,24 @@ def check(fn: str, code: int) -> None:
         if code == ERR_ARG:
             raise ValueError(f"{fn}: {msg}")
         raise GmvsError(fn, code, msg)
+
+
+def profile_enable(on: bool = True) -> None:
+    check("gmvs_profile_enable", lib.gmvs_profile_enable(1 if on else 0))
+
+
+def profile_reset() -> None:
+    check("gmvs_profile_reset", lib.gmvs_profile_reset())
+
+
+def profile_collect() -> dict:
+    """{kernel name: (launches, total ms)} accumulated since the last reset."""
+    n = ctypes.c_int32(0)
+    check("gmvs_profile_collect", lib.gmvs_profile_collect(None, 0, ctypes.byref(n)))
+    arr = (CKernelStat * max(1, n.value))()
+    check("gmvs_profile_collect", lib.gmvs_profile_collect(arr, n.value, ctypes.byref(n)))
+    return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms))
+            for i in range(n.value)}
 
 
 def ptr_array(ptrs: Sequence[Optional[int]]):

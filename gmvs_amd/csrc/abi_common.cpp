@@ -3,6 +3,8 @@
 #include "abi_common.h"
 
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "pm_device.h"
 
@@ -23,7 +25,126 @@ void set_error(const char* fmt, ...) {
 
 void clear_error() { g_error.clear(); }
 
+namespace {
+
+const char* const kKernelNames[K_NUM] = {
+    "pm_setup_view", "pm_pack_quads", "pm_init",      "pm_candidates",  "pm_select_refine",
+    "pm_depth_normal", "pm_filter",   "pm_export",    "pm_cost_batch",  "fuse_fill",
+    "fuse_check",     "fuse_scatter", "fuse_write_view"};
+
+struct Pending {
+    int kid;
+    hipEvent_t a, b;
+};
+
+struct Profiler {
+    std::mutex mu;
+    bool on = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[K_NUM] = {};
+    long long count[K_NUM] = {};
+
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+Profiler& prof() {
+    static Profiler p;
+    return p;
+}
+
+thread_local hipEvent_t t_start = nullptr;
+
+}  // namespace
+
+bool profiling_enabled() { return prof().on; }
+
+void profile_begin(hipStream_t st, int kid) {
+    Profiler& p = prof();
+    if (!p.on) return;
+    std::lock_guard<std::mutex> lk(p.mu);
+    t_start = p.take();
+    if (t_start && hipEventRecord(t_start, st) != hipSuccess) {
+        p.pool.push_back(t_start);
+        t_start = nullptr;
+    }
+}
+
+void profile_end(hipStream_t st, int kid) {
+    Profiler& p = prof();
+    if (!p.on || !t_start) return;
+    std::lock_guard<std::mutex> lk(p.mu);
+    hipEvent_t e = p.take();
+    if (e && hipEventRecord(e, st) == hipSuccess) {
+        p.pending.push_back(Pending{kid, t_start, e});
+    } else {
+        p.pool.push_back(t_start);
+        if (e) p.pool.push_back(e);
+    }
+    t_start = nullptr;
+}
+
 }  // namespace gmvs
+
+extern "C" int gmvs_profile_enable(int32_t on) {
+    gmvs::Profiler& p = gmvs::prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.on = on != 0;
+    return GMVS_OK;
+}
+
+extern "C" int gmvs_profile_collect(gmvs_kernel_stat* out, int32_t max_out, int32_t* num_out) {
+    gmvs::clear_error();
+    gmvs::Profiler& p = gmvs::prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    for (const gmvs::Pending& q : p.pending) {
+        float ms = 0.f;
+        GMVS_HIP_CHECK(hipEventSynchronize(q.b));
+        GMVS_HIP_CHECK(hipEventElapsedTime(&ms, q.a, q.b));
+        p.ms[q.kid] += ms;
+        p.count[q.kid] += 1;
+        p.pool.push_back(q.a);
+        p.pool.push_back(q.b);
+    }
+    p.pending.clear();
+    int n = 0;
+    for (int k = 0; k < gmvs::K_NUM; ++k) {
+        if (p.count[k] == 0) continue;
+        if (out && n < max_out) {
+            std::snprintf(out[n].name, sizeof(out[n].name), "%s", gmvs::kKernelNames[k]);
+            out[n].launches = p.count[k];
+            out[n].total_ms = p.ms[k];
+        }
+        ++n;
+    }
+    if (num_out) *num_out = n;
+    return GMVS_OK;
+}
+
+extern "C" int gmvs_profile_reset(void) {
+    gmvs::Profiler& p = gmvs::prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    for (const gmvs::Pending& q : p.pending) {
+        (void)hipEventSynchronize(q.b);
+        p.pool.push_back(q.a);
+        p.pool.push_back(q.b);
+    }
+    p.pending.clear();
+    for (int k = 0; k < gmvs::K_NUM; ++k) {
+        p.ms[k] = 0.0;
+        p.count[k] = 0;
+    }
+    return GMVS_OK;
+}
 
 extern "C" const char* gmvs_last_error(void) { return gmvs::g_error.c_str(); }
 

@@ -14,7 +14,47 @@ namespace gmvs {
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 void clear_error();
 
+// Opt-in kernel timing: when enabled, every launch of the library is
+// bracketed by a pair of hipEvents recorded on its stream; collecting waits
+// for them and sums the elapsed time per kernel name.
+enum KernelId {
+    K_SETUP_VIEW = 0,
+    K_PACK,
+    K_INIT,
+    K_CANDIDATES,
+    K_SELECT_REFINE,
+    K_DEPTH_NORMAL,
+    K_FILTER,
+    K_EXPORT,
+    K_COST_BATCH,
+    K_FUSE_FILL,
+    K_FUSE_CHECK,
+    K_FUSE_SCATTER,
+    K_FUSE_VIEWDESC,
+    K_NUM
+};
+
+bool profiling_enabled();
+void profile_begin(hipStream_t st, int kid);
+void profile_end(hipStream_t st, int kid);
+
+struct ProfScope {
+    hipStream_t st;
+    int kid;
+    ProfScope(hipStream_t s, int k) : st(s), kid(k) { profile_begin(st, kid); }
+    ~ProfScope() { profile_end(st, kid); }
+};
+
 }  // namespace gmvs
+
+#define GMVS_PROF(st, kid) gmvs::ProfScope gmvs_prof_scope_(st, kid)
+
+// hipLaunchKernelGGL bracketed by the (opt-in) timing events.
+#define GMVS_LAUNCH(kid, st, ...)                    \
+    do {                                             \
+        gmvs::ProfScope gmvs_prof_scope_(st, kid);   \
+        hipLaunchKernelGGL(__VA_ARGS__);             \
+    } while (0)
 
 #define GMVS_HIP_CHECK(expr)                                                                  \
     do {                                                                                      \

@@ -270,7 +270,7 @@ extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
         fv.depth = depths[i];
         fv.normal = normals[i];
         fv.global_id = i ? src_ids[i - 1] : ref_id;
-        hipLaunchKernelGGL(fuse_write_view, dim3(1), dim3(64), 0, st, fv, vtab + i);
+        GMVS_LAUNCH(K_FUSE_VIEWDESC, st, fuse_write_view, dim3(1), dim3(64), 0, st, fv, vtab + i);
         GMVS_LAUNCH_CHECK();
     }
     const int hw = h * w;
@@ -279,15 +279,15 @@ extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
     int32_t* winner = reinterpret_cast<int32_t*>(ws + L.winner);
     int32_t* rec_idx = reinterpret_cast<int32_t*>(ws + L.rec_idx);
     float4* rec_val = reinterpret_cast<float4*>(ws + L.rec_val);
-    hipLaunchKernelGGL(fuse_fill, grid, block, 0, st, hw, out_depth, out_normal, winner,
+    GMVS_LAUNCH(K_FUSE_FILL, st, fuse_fill, grid, block, 0, st, hw, out_depth, out_normal, winner,
                        masks + (size_t)ref_id * hw, mask_copy);
     GMVS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(fuse_check, grid, block, 0, st,
+    GMVS_LAUNCH(K_FUSE_CHECK, st, fuse_check, grid, block, 0, st,
                        reinterpret_cast<const FuseView*>(ws + L.views), num_images - 1, h, w,
                        geom_consistent, reproj_px, rel_depth, cos_angle, mask_copy, masks, winner,
                        rec_idx, rec_val);
     GMVS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(fuse_scatter, grid, block, 0, st, hw, winner, rec_idx, rec_val, out_depth,
+    GMVS_LAUNCH(K_FUSE_SCATTER, st, fuse_scatter, grid, block, 0, st, hw, winner, rec_idx, rec_val, out_depth,
                        out_normal);
     GMVS_LAUNCH_CHECK();
     return GMVS_OK;

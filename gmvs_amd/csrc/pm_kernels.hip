@@ -152,8 +152,21 @@ __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict_
         pl.z = rc.R[6] * n0 + rc.R[7] * n1 + rc.R[8] * n2;
         pl.w = dist_to_origin(rc, fpx, fpy, P.init_depth[c], pl);
     } else if (P.init_mode == GMVS_INIT_PRIOR) {
+        // Reuse the previous plane (its .w holds a depth), then overwrite it
+        // with a perturbed prior plane for masked, costly pixels. Written as
+        // straight-line code followed by one branch: the nested short-circuit
+        // form (prior && mask && cost) was miscompiled by the ROCm 7.2 gfx950
+        // backend (the reuse path read registers clobbered by the mask load).
         const float4 st = planes[c];
-        if (P.prior && P.plane_masks[c] > 0u && costs[c] >= 0.1f) {
+        pl = st;
+        pl.w = dist_to_origin(rc, fpx, fpy, st.w, st);
+        bool perturb = false;
+        if (P.prior) {
+            const uint32_t mval = P.plane_masks[c];
+            const float cval = costs[c];
+            perturb = (mval > 0u) & (cval >= 0.1f);
+        }
+        if (perturb) {
             const float4 pp = P.prior_planes[c];
             const float pert = 0.02f;
             const float dlo = (1.0f - 3.0f * pert) * pp.w;
@@ -164,9 +177,6 @@ __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict_
             pl = perturbed_normal(rc, fpx, fpy, pp, u01(o[1]), u01(o[2]), u01(o[3]),
                                   3.0f * pert * kPiF);
             pl.w = dp;
-        } else {
-            pl = st;
-            pl.w = dist_to_origin(rc, fpx, fpy, st.w, st);
         }
     } else {
         const float depth = rng.u(RNG_INIT_DEPTH) * (P.dmax - P.dmin) + P.dmin;
@@ -475,7 +485,11 @@ __device__ inline float prior_score(const PMConst& P, float cost, float depth, f
     const float asig = kPiF * (5.0f / 180.0f);
     const float two_as2 = 2.0f * asig * asig;
     const float dd = depth - dprior;
-    const float ang = acosf(pp.x * pl.x + pp.y * pl.y + pp.z * pl.z);
+    // unit-normal dot clamped into [-1, 1] (NaN stays NaN), so identical
+    // normals give angle 0 rather than acos's NaN just above 1
+    float dot = pp.x * pl.x + pp.y * pl.y + pp.z * pl.z;
+    dot = dot > 1.0f ? 1.0f : (dot < -1.0f ? -1.0f : dot);
+    const float ang = acosf(dot);
     const float pr = 0.5f + expf(-dd * dd / two_ds2) * expf(-ang * ang / two_as2);
     return expf(-cost * cost / 0.18f) * pr;
 }
@@ -933,10 +947,10 @@ int setup_views(int num_images, const gmvs_camera* cams, const float* const* ima
     ViewDesc* views = reinterpret_cast<ViewDesc*>(ws + L.views);
     const int blocks = int((hw + 255) / 256);
     for (int v = 0; v < nv; ++v) {
-        hipLaunchKernelGGL(pm_setup_view, dim3(1), dim3(64), 0, st, cams[0], cams[v + 1],
+        GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_view, dim3(1), dim3(64), 0, st, cams[0], cams[v + 1],
                            quads + hw * v, depths ? depths[v + 1] : nullptr, views + v);
         GMVS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(pm_pack_quads, dim3(blocks), dim3(256), 0, st, images[v + 1],
+        GMVS_LAUNCH(K_PACK, st, pm_pack_quads, dim3(blocks), dim3(256), 0, st, images[v + 1],
                            quads + hw * v, h, w);
         GMVS_LAUNCH_CHECK();
     }
@@ -1053,33 +1067,33 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
     const int na = h * P.wh;
 
     const int tiles_full = ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
-    hipLaunchKernelGGL(pm_init<6>, dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
+    GMVS_LAUNCH(K_INIT, st, pm_init<6>, dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
                        A.costs, sel);
     GMVS_LAUNCH_CHECK();
     const int tiles_half = ((P.wh + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
     const dim3 grid_half(grid_for_tiles(tiles_half));
     for (int it = 0; it < A.params.max_iterations; ++it) {
         for (int color = 0; color < 2; ++color) {
-            hipLaunchKernelGGL(pm_candidates<6>, grid_half, dim3(kBlock), 0, st, P, color, planes,
+            GMVS_LAUNCH(K_CANDIDATES, st, pm_candidates<6>, grid_half, dim3(kBlock), 0, st, P, color, planes,
                                A.costs, ca, cplane, cflag, na);
             GMVS_LAUNCH_CHECK();
-            hipLaunchKernelGGL(pm_select_refine<6>, grid_half, dim3(kBlock), 0, st, P, color, it,
+            GMVS_LAUNCH(K_SELECT_REFINE, st, pm_select_refine<6>, grid_half, dim3(kBlock), 0, st, P, color, it,
                                planes, A.costs, sel, ca, cplane, cflag, na);
             GMVS_LAUNCH_CHECK();
         }
     }
     const int hw = h * w;
-    hipLaunchKernelGGL(pm_depth_normal, dim3((hw + 255) / 256), dim3(256), 0, st, P, planes);
+    GMVS_LAUNCH(K_DEPTH_NORMAL, st, pm_depth_normal, dim3((hw + 255) / 256), dim3(256), 0, st, P, planes);
     GMVS_LAUNCH_CHECK();
     if (!A.skip_filter) {
         for (int color = 0; color < 2; ++color) {
-            hipLaunchKernelGGL(pm_filter, dim3((na + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P,
+            GMVS_LAUNCH(K_FILTER, st, pm_filter, dim3((na + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P,
                                color, planes, A.costs);
             GMVS_LAUNCH_CHECK();
         }
     }
     if (A.out_depth || A.out_normal || A.out_cost) {
-        hipLaunchKernelGGL(pm_export, dim3((hw + 255) / 256), dim3(256), 0, st, hw, planes, A.costs,
+        GMVS_LAUNCH(K_EXPORT, st, pm_export, dim3((hw + 255) / 256), dim3(256), 0, st, hw, planes, A.costs,
                            A.out_depth, A.out_normal, A.out_cost);
         GMVS_LAUNCH_CHECK();
     }
@@ -1117,7 +1131,7 @@ extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
     if (int rc = setup_views(num_images, cameras, images, nullptr, ws, L, st)) return rc;
     PMConst P = make_const(num_images, cameras, images[0],
                            reinterpret_cast<const ViewDesc*>(ws + L.views), *params, radius);
-    hipLaunchKernelGGL(pm_cost_batch<6>, dim3((n + 255) / 256), dim3(256), 0, st, P, n, xy,
+    GMVS_LAUNCH(K_COST_BATCH, st, pm_cost_batch<6>, dim3((n + 255) / 256), dim3(256), 0, st, P, n, xy,
                        reinterpret_cast<const float4*>(planes), costs);
     GMVS_LAUNCH_CHECK();
     return GMVS_OK;

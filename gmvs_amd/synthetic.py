@@ -296,15 +296,12 @@ def pair_sources(cams: List[Camera], max_src: int) -> List[List[int]]:
 
 
 def voronoi_labels(width: int, height: int, cells: int, seed: int) -> np.ndarray:
+    """int32 labels 1..cells of a seeded Voronoi partition of the image."""
+    from scipy.spatial import cKDTree
+
     gen = np.random.default_rng(seed)
-    sx = gen.uniform(0, width, cells)
-    sy = gen.uniform(0, height, cells)
+    pts = np.stack([gen.uniform(0, width, cells), gen.uniform(0, height, cells)], 1)
     ys, xs = np.mgrid[0:height, 0:width]
-    best = np.full((height, width), np.inf)
-    lab = np.zeros((height, width), np.int32)
-    for i in range(cells):
-        d2 = (xs - sx[i]) ** 2 + (ys - sy[i]) ** 2
-        m = d2 < best
-        lab[m] = i + 1
-        best = np.minimum(best, d2)
-    return lab
+    q = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.float64)
+    _, idx = cKDTree(pts).query(q, k=1)
+    return (idx.astype(np.int32) + 1).reshape(height, width)

@@ -180,6 +180,21 @@ int gmvs_prior_build(const gmvs_camera* ref_camera, const float* depth, const fl
 int gmvs_delaunay(const int32_t* xy, int32_t n, int32_t* tri_out, int32_t max_tri,
                   int32_t* num_tri);
 
+/* Kernel timing. No counterpart in the reference (which only prints
+ * wall-clock totals, gmvs/scripts/launch.py:125-126,176-177). When enabled,
+ * every kernel launch of the library is bracketed by hipEvents on its
+ * stream; gmvs_profile_collect waits for the recorded events and returns the
+ * accumulated launches and milliseconds per kernel name (out may be NULL to
+ * query the count). */
+typedef struct gmvs_kernel_stat {
+    char name[32];
+    int64_t launches;
+    double total_ms;
+} gmvs_kernel_stat;
+int gmvs_profile_enable(int32_t on);
+int gmvs_profile_collect(gmvs_kernel_stat* out, int32_t max_out, int32_t* num_out);
+int gmvs_profile_reset(void);
+
 /* Philox4x32-10 block (host), exported for known-answer tests. */
 void gmvs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -659,7 +659,14 @@ struct Run {
         const T asig = T(kPi) * (T(5) / T(180));
         const T two_as2 = T(2) * asig * asig;
         const T dd = depth_v - depth_prior;
-        const T ang = std::acos(pp.x * pl.x + pp.y * pl.y + pp.z * pl.z);
+        // The dot product of two unit normals is clamped into [-1, 1] (NaN
+        // stays NaN): the reference's acos of the raw dot (lib 1266, 1276,
+        // 789) returns NaN for identical normals whenever rounding lands just
+        // above 1, which silently drops the candidate in one precision and
+        // not in another.
+        T dot = pp.x * pl.x + pp.y * pl.y + pp.z * pl.z;
+        dot = dot > T(1) ? T(1) : (dot < T(-1) ? T(-1) : dot);
+        const T ang = std::acos(dot);
         const T pr = gamma + std::exp(-dd * dd / two_ds2) * std::exp(-ang * ang / two_as2);
         return std::exp(-cost_v * cost_v / beta) * pr;
     }

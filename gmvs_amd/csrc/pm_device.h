@@ -265,24 +265,59 @@ __device__ inline float geom_cost(const RefCam& rc, const ViewDesc& vd, float4 p
 }
 
 // ------------------------------------------------------------------ NCC ---
-// Per-pixel hoist of the reference side of the bilateral NCC
-// (compute_bilateral_NCC, src 362-445): the NT*NT taps' weights w and
-// w * (r - c) with c the centre intensity, and the weighted mean/variance of
-// the reference patch from shifted (centre-relative) moments.
-template <int NT>
+// The bilateral NCC (compute_bilateral_NCC, src 362-445) is evaluated by a
+// group of L consecutive lanes per reference pixel (L = 1, 2 or 4). The
+// NT x NT taps (i along x, j along y) are split into blocks of NR rows x NC
+// columns, one block per lane: L = 1 takes all taps, L = 2 takes every other
+// row, L = 4 takes a quadrant. Each lane hoists the reference side of its
+// taps once per pixel -- the bilateral weight w and w * (r - c), c being the
+// centre intensity -- and the group reduces its partial sums with DPP quad
+// permutes (a fixed, commutative order, so every lane holds the same bits).
+// Moments are accumulated about c (shift-invariant, no E[x^2] - E[x]^2
+// cancellation).
+template <int L>
+__device__ inline float group_sum(float v) {
+    static_assert(L == 1 || L == 2 || L == 4, "lane group of 1, 2 or 4");
+    if constexpr (L >= 2)  // quad_perm [1,0,3,2]
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+    if constexpr (L >= 4)  // quad_perm [2,3,0,1]
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+    return v;
+}
+
+template <int NT, int L>
+struct TapBlock {
+    static constexpr int NR = (L == 1) ? NT : NT / 2;        // rows (x offsets) per lane
+    static constexpr int NC = (L == 4) ? NT / 2 : NT;        // columns (y offsets) per lane
+    static_assert(NT % 2 == 0 || L == 1, "split groups need an even tap count");
+    // row index of the lane's r-th row, column index of its c-th column
+    __device__ static int row(int sub, int r) {
+        if constexpr (L == 1) return r;
+        else if constexpr (L == 2) return sub + 2 * r;
+        else return (sub >> 1) * NR + r;
+    }
+    __device__ static int col(int sub, int c) {
+        if constexpr (L == 4) return (sub & 1) * NC + c;
+        else return c;
+    }
+};
+
+template <int NT, int L>
 struct Hoist {
-    float wt[NT * NT];
-    float wr[NT * NT];
-    float tx[NT];  // clamped tap columns (reference taps, src 400-405)
-    float ty[NT];  // clamped tap rows
+    using B = TapBlock<NT, L>;
+    float wt[B::NR * B::NC];
+    float wr[B::NR * B::NC];
+    float tx[B::NR];  // clamped tap columns of the lane's rows (src 400-405)
+    float ty[B::NC];  // clamped tap rows of the lane's columns
     float inv_w, mean_r, var_r, c;
     float px, py;
 };
 
-template <int NT>
-__device__ inline void hoist_ref(Hoist<NT>& hs, const float* __restrict__ ref, int px, int py,
+template <int NT, int L>
+__device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref, int px, int py,
                                  int h, int w, int radius, int inc, float sigma_spatial,
-                                 float sigma_color) {
+                                 float sigma_color, int sub) {
+    using B = TapBlock<NT, L>;
     hs.px = (float)px;
     hs.py = (float)py;
     const float c = ref[py * w + px];
@@ -290,69 +325,97 @@ __device__ inline void hoist_ref(Hoist<NT>& hs, const float* __restrict__ ref, i
     const float ss = 2.0f * sigma_spatial * sigma_spatial;
     const float sc = 2.0f * sigma_color * sigma_color;
 #pragma unroll
-    for (int i = 0; i < NT; ++i) {
-        hs.tx[i] = (float)min(max(px - radius + i * inc, 0), w - 1);
-        hs.ty[i] = (float)min(max(py - radius + i * inc, 0), h - 1);
-    }
+    for (int r = 0; r < B::NR; ++r)
+        hs.tx[r] = (float)min(max(px - radius + B::row(sub, r) * inc, 0), w - 1);
+#pragma unroll
+    for (int q = 0; q < B::NC; ++q)
+        hs.ty[q] = (float)min(max(py - radius + B::col(sub, q) * inc, 0), h - 1);
     float W = 0.f, R1 = 0.f, R2 = 0.f;
 #pragma unroll
-    for (int i = 0; i < NT; ++i) {
-        const float di = (float)(-radius + i * inc);
+    for (int r = 0; r < B::NR; ++r) {
+        const float di = (float)(-radius + B::row(sub, r) * inc);
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            const float dj = (float)(-radius + j * inc);
-            const float r = ref[(int)hs.ty[j] * w + (int)hs.tx[i]] - c;
-            const float wgt = expf(-sqrtf(di * di + dj * dj) / ss - fabsf(r) / sc);
-            hs.wt[i * NT + j] = wgt;
-            hs.wr[i * NT + j] = wgt * r;
+        for (int q = 0; q < B::NC; ++q) {
+            const float dj = (float)(-radius + B::col(sub, q) * inc);
+            const float rv = ref[(int)hs.ty[q] * w + (int)hs.tx[r]] - c;
+            const float wgt = expf(-sqrtf(di * di + dj * dj) / ss - fabsf(rv) / sc);
+            hs.wt[r * B::NC + q] = wgt;
+            hs.wr[r * B::NC + q] = wgt * rv;
             W += wgt;
-            R1 = fmaf(wgt, r, R1);
-            R2 = fmaf(wgt * r, r, R2);
+            R1 = fmaf(wgt, rv, R1);
+            R2 = fmaf(wgt * rv, rv, R2);
         }
     }
+    W = group_sum<L>(W);
+    R1 = group_sum<L>(R1);
+    R2 = group_sum<L>(R2);
     hs.inv_w = 1.0f / W;
     hs.mean_r = R1 * hs.inv_w;
     hs.var_r = R2 * hs.inv_w - hs.mean_r * hs.mean_r;
 }
 
+// Device buffers are global memory; loading through an address-space-1
+// pointer lets the compiler emit global (not flat) loads, which only count
+// against vmcnt and so can stay in flight across LDS traffic.
+typedef float __attribute__((ext_vector_type(4))) f32x4;
+typedef const f32x4 __attribute__((address_space(1))) gfloat4;
+
+__device__ inline gfloat4* as_global(const float4* p) { return (gfloat4*)p; }
+
 // One bilateral NCC of the hoisted pixel against one source view. The
 // source image is stored as packed bilinear cells (v00, v01-v00, v10-v00,
 // v11-v10-v01+v00) with the reference's edge clamping already applied, so a
-// tap is one 16-byte gather and three FMAs.
-template <int NT>
-__device__ inline float ncc_eval(const Hoist<NT>& hs, const float H[9],
-                                 const float4* __restrict__ quad, int h, int w) {
+// tap is one 16-byte gather and three FMAs. Each lane issues the gathers of
+// a whole patch row before consuming them. All lanes of a group return the
+// same cost.
+template <int NT, int L>
+__device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9],
+                                 const float4* __restrict__ quad_ptr, int h, int w) {
+    using B = TapBlock<NT, L>;
     const float cz = fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8]));
     const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) / cz;
     const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) / cz;
     if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
+    gfloat4* quad = as_global(quad_ptr);
     const float wm1 = (float)(w - 1), hm1 = (float)(h - 1);
+    const uint32_t uw = (uint32_t)w;
     float S1 = 0.f, S2 = 0.f, X = 0.f;
 #pragma unroll
-    for (int i = 0; i < NT; ++i) {
-        const float au = fmaf(H[0], hs.tx[i], H[2]);
-        const float av = fmaf(H[3], hs.tx[i], H[5]);
-        const float az = fmaf(H[6], hs.tx[i], H[8]);
+    for (int r = 0; r < B::NR; ++r) {
+        const float au = fmaf(H[0], hs.tx[r], H[2]);
+        const float av = fmaf(H[3], hs.tx[r], H[5]);
+        const float az = fmaf(H[6], hs.tx[r], H[8]);
+        f32x4 q[B::NC];
+        float fx[B::NC], fy[B::NC];
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
+        for (int j = 0; j < B::NC; ++j) {
             const float u = fmaf(H[1], hs.ty[j], au);
             const float v = fmaf(H[4], hs.ty[j], av);
             const float z = fmaf(H[7], hs.ty[j], az);
             const float rz = __builtin_amdgcn_rcpf(z);
             const float x = u * rz, y = v * rz;
-            const float xf = fminf(fmaxf(floorf(x), 0.f), wm1);
-            const float yf = fminf(fmaxf(floorf(y), 0.f), hm1);
-            const float fx = x - xf, fy = y - yf;
-            const float4 q = quad[(int)yf * w + (int)xf];
-            const float top = fmaf(fx, q.y, q.x);
-            const float dyv = fmaf(fx, q.w, q.z);
-            const float s = fmaf(fy, dyv, top) - hs.c;
-            const float ws = hs.wt[i * NT + j] * s;
+            // clamp(trunc(c), 0, n-1) of the reference equals
+            // clamp(floor(c), 0, n-1); any NaN/inf lands inside [0, n-1]
+            const float xf = __builtin_amdgcn_fmed3f(floorf(x), 0.f, wm1);
+            const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
+            fx[j] = x - xf;
+            fy[j] = y - yf;
+            q[j] = quad[__umul24((uint32_t)yf, uw) + (uint32_t)xf];
+        }
+#pragma unroll
+        for (int j = 0; j < B::NC; ++j) {
+            const float top = fmaf(fx[j], q[j].y, q[j].x);
+            const float dyv = fmaf(fx[j], q[j].w, q[j].z);
+            const float s = fmaf(fy[j], dyv, top) - hs.c;
+            const float ws = hs.wt[r * B::NC + j] * s;
             S1 += ws;
             S2 = fmaf(ws, s, S2);
-            X = fmaf(hs.wr[i * NT + j], s, X);
+            X = fmaf(hs.wr[r * B::NC + j], s, X);
         }
     }
+    S1 = group_sum<L>(S1);
+    S2 = group_sum<L>(S2);
+    X = group_sum<L>(X);
     const float ms = S1 * hs.inv_w;
     const float var_s = S2 * hs.inv_w - ms * ms;
     if (hs.var_r < 1e-5f || var_s < 1e-5f) return 2.0f;

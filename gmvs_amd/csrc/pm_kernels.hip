@@ -27,9 +27,25 @@
 
 namespace gmvs {
 
-constexpr int kTile = 16;  // 16 x 16 threads per block
-constexpr int kBlock = kTile * kTile;
-constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kBlock = 256;  // threads per block
+constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// Lane groups of L lanes evaluate one pixel's NCCs together. Per block
+// 256 / L pixels in a TX x TY tile (TX counts active columns in the
+// half-sweep kernels and pixel columns elsewhere).
+template <int L>
+struct Tiling {
+    static constexpr int PIX = kBlock / L;
+    static constexpr int TX = (L == 4) ? 8 : 16;
+    static constexpr int TY = PIX / TX;
+    __host__ __device__ static int tiles(int cols, int rows) {
+        return ((cols + TX - 1) / TX) * ((rows + TY - 1) / TY);
+    }
+};
+
+// The NCC lane-group width used by the kernels.
+constexpr int kLanes = 4;
+constexpr int kNT = 6;  // taps per axis: patch 11, increment 2 (the reference defaults)
 
 struct PMConst {
     RefCam rc;
@@ -128,16 +144,18 @@ __global__ void pm_pack_quads(const float* __restrict__ img, float4* __restrict_
 }
 
 // ---------------------------------------------------------------- init ---
-template <int NT>
+template <int NT, int L>
 __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict__ planes,
                                                   float* __restrict__ costs,
                                                   uint32_t* __restrict__ sel) {
+    using TL = Tiling<L>;
     __shared__ float s_c[kMaxViews][kBlock];
     int tile;
-    if (!tile_of_block(((P.w + kTile - 1) / kTile) * ((P.h + kTile - 1) / kTile), tile)) return;
-    const int tx = (P.w + kTile - 1) / kTile;
-    const int px = (tile % tx) * kTile + (threadIdx.x & (kTile - 1));
-    const int py = (tile / tx) * kTile + (threadIdx.x / kTile);
+    if (!tile_of_block(TL::tiles(P.w, P.h), tile)) return;
+    const int tx = (P.w + TL::TX - 1) / TL::TX;
+    const int slot = threadIdx.x / L, sub = threadIdx.x % L;
+    const int px = (tile % tx) * TL::TX + (slot % TL::TX);
+    const int py = (tile / tx) * TL::TY + (slot / TL::TX);
     if (px >= P.w || py >= P.h) return;
     const int c = py * P.w + px;
     const RefCam& rc = P.rc;
@@ -184,8 +202,8 @@ __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict_
         pl.w = dist_to_origin(rc, fpx, fpy, depth, pl);
     }
 
-    Hoist<NT> hs;
-    hoist_ref<NT>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+    Hoist<NT, L> hs;
+    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub);
     float m[3];
     plane_m(rc, pl, m);
     int valid = 0;
@@ -193,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict_
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+        const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
         s_c[v][threadIdx.x] = cv;
         valid += (cv < 2.0f);
     }
@@ -225,22 +243,28 @@ __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict_
             if (s_c[v][threadIdx.x] <= thr) bits |= 1u << v;
         cost = sum / (float)k;
     }
-    planes[c] = pl;
-    costs[c] = cost;
-    sel[c] = bits;
+    if (sub == 0) {
+        planes[c] = pl;
+        costs[c] = cost;
+        sel[c] = bits;
+    }
 }
 
 // ------------------------------------------------------- half-sweep, A ---
 struct Active {
-    int px, py, aidx;
+    int px, py, aidx, sub;
     bool ok;
 };
 
+template <int L>
 __device__ inline Active active_pixel(const PMConst& P, int tile, int color) {
-    const int tx = (P.wh + kTile - 1) / kTile;
-    const int ax = (tile % tx) * kTile + (threadIdx.x & (kTile - 1));
-    const int y = (tile / tx) * kTile + (threadIdx.x / kTile);
+    using TL = Tiling<L>;
+    const int tx = (P.wh + TL::TX - 1) / TL::TX;
+    const int slot = threadIdx.x / L;
+    const int ax = (tile % tx) * TL::TX + (slot % TL::TX);
+    const int y = (tile / tx) * TL::TY + (slot / TL::TX);
     Active a;
+    a.sub = threadIdx.x % L;
     a.py = y;
     a.px = 2 * ax + ((y + color) & 1);
     a.aidx = y * P.wh + ax;
@@ -419,7 +443,7 @@ __device__ inline void candidate_search(const float* __restrict__ cs, int px, in
     }
 }
 
-template <int NT>
+template <int NT, int L>
 __global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
                                                         const float4* __restrict__ planes,
                                                         const float* __restrict__ costs,
@@ -427,26 +451,28 @@ __global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
                                                         float4* __restrict__ cplane,
                                                         uint8_t* __restrict__ cflag, int NA) {
     int tile;
-    if (!tile_of_block(((P.wh + kTile - 1) / kTile) * ((P.h + kTile - 1) / kTile), tile)) return;
-    const Active a = active_pixel(P, tile, color);
+    if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
+    const Active a = active_pixel<L>(P, tile, color);
     if (!a.ok) return;
+    const bool lead = a.sub == 0;
     uint32_t flags;
     int pos[8];
     candidate_search(costs, a.px, a.py, P.h, P.w, flags, pos);
-    cflag[a.aidx] = (uint8_t)flags;
-    Hoist<NT> hs;
-    hoist_ref<NT>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+    if (lead) cflag[a.aidx] = (uint8_t)flags;
+    Hoist<NT, L> hs;
+    hoist_ref<NT, L>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub);
     for (int k = 0; k < 8; ++k) {
         if (!((flags >> k) & 1u)) continue;
         const float4 pl = planes[pos[k]];
-        cplane[(size_t)k * NA + a.aidx] = pl;
+        if (lead) cplane[(size_t)k * NA + a.aidx] = pl;
         float m[3];
         plane_m(P.rc, pl, m);
         for (int v = 0; v < P.nv; ++v) {
             const ViewDesc& vd = P.views[v];
             float H[9];
             homography(vd.A, vd.b, m, H);
-            ca[((size_t)k * P.nv + v) * NA + a.aidx] = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+            const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+            if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
     }
 }
@@ -502,7 +528,7 @@ __device__ inline float ca_entry(const float* __restrict__ ca, uint32_t flags, i
     return (k == 0 && v == 0) ? 2.0f : 0.0f;
 }
 
-template <int NT>
+template <int NT, int L>
 __global__ __launch_bounds__(kBlock) void pm_select_refine(
     PMConst P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
     uint32_t* __restrict__ sel, const float* __restrict__ ca, const float4* __restrict__ cplane,
@@ -517,8 +543,8 @@ __global__ __launch_bounds__(kBlock) void pm_select_refine(
         __syncthreads();
     }
     int tile;
-    if (!tile_of_block(((P.wh + kTile - 1) / kTile) * ((P.h + kTile - 1) / kTile), tile)) return;
-    const Active a = active_pixel(P, tile, color);
+    if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
+    const Active a = active_pixel<L>(P, tile, color);
     if (!a.ok) return;
     const int px = a.px, py = a.py, aidx = a.aidx;
     const int c = py * P.w + px;
@@ -631,8 +657,8 @@ __global__ __launch_bounds__(kBlock) void pm_select_refine(
     }
     const int kmin = argmin_last8(fc);
 
-    Hoist<NT> hs;
-    hoist_ref<NT>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+    Hoist<NT, L> hs;
+    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub);
 
     // Weighted cost of a hypothesis over the selected views (+ geometric term).
     auto wcost = [&](float4 pl) -> float {
@@ -644,7 +670,7 @@ __global__ __launch_bounds__(kBlock) void pm_select_refine(
             const ViewDesc& vd = s_views[v];
             float H[9];
             homography(vd.A, vd.b, m, H);
-            const float x = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+            const float x = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
             const float wv = weight_of(v);
             if (P.geom)
                 s += wv * (x + 0.1f * geom_cost(rc, vd, pl, fpx, fpy));
@@ -747,9 +773,11 @@ __global__ __launch_bounds__(kBlock) void pm_select_refine(
             cost = tc;
         }
     }
-    planes[c] = cur;
-    costs[c] = cost;
-    sel[c] = selc;
+    if (a.sub == 0) {
+        planes[c] = cur;
+        costs[c] = cost;
+        sel[c] = selc;
+    }
 }
 
 // ------------------------------------------------------ finalisation ---
@@ -857,25 +885,30 @@ __global__ void pm_export(int hw, const float4* __restrict__ planes, const float
 }
 
 // Parity hook: V NCCs for a list of (pixel, plane) pairs.
-template <int NT>
-__global__ __launch_bounds__(256) void pm_cost_batch(PMConst P, int n, const int32_t* __restrict__ xy,
-                              const float4* __restrict__ planes, float* __restrict__ out) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+template <int NT, int L>
+__global__ __launch_bounds__(kBlock) void pm_cost_batch(PMConst P, int n,
+                                                        const int32_t* __restrict__ xy,
+                                                        const float4* __restrict__ planes,
+                                                        float* __restrict__ out) {
+    const int q = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+    const int sub = threadIdx.x % L;
     if (q >= n) return;
     const int px = xy[2 * q], py = xy[2 * q + 1];
     if (px < 0 || px >= P.w || py < 0 || py >= P.h) {
-        for (int v = 0; v < P.nv; ++v) out[(size_t)q * P.nv + v] = __builtin_nanf("");
+        if (sub == 0)
+            for (int v = 0; v < P.nv; ++v) out[(size_t)q * P.nv + v] = __builtin_nanf("");
         return;
     }
-    Hoist<NT> hs;
-    hoist_ref<NT>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c);
+    Hoist<NT, L> hs;
+    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub);
     float m[3];
     plane_m(P.rc, planes[q], m);
     for (int v = 0; v < P.nv; ++v) {
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        out[(size_t)q * P.nv + v] = ncc_eval<NT>(hs, H, vd.quad, P.h, P.w);
+        const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+        if (sub == 0) out[(size_t)q * P.nv + v] = cv;
     }
 }
 
@@ -1066,18 +1099,18 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
     uint8_t* cflag = reinterpret_cast<uint8_t*>(ws + L.cflag);
     const int na = h * P.wh;
 
-    const int tiles_full = ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
-    GMVS_LAUNCH(K_INIT, st, pm_init<6>, dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
+    const int tiles_full = Tiling<kLanes>::tiles(w, h);
+    GMVS_LAUNCH(K_INIT, st, (pm_init<kNT, kLanes>), dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
                        A.costs, sel);
     GMVS_LAUNCH_CHECK();
-    const int tiles_half = ((P.wh + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
+    const int tiles_half = Tiling<kLanes>::tiles(P.wh, h);
     const dim3 grid_half(grid_for_tiles(tiles_half));
     for (int it = 0; it < A.params.max_iterations; ++it) {
         for (int color = 0; color < 2; ++color) {
-            GMVS_LAUNCH(K_CANDIDATES, st, pm_candidates<6>, grid_half, dim3(kBlock), 0, st, P, color, planes,
+            GMVS_LAUNCH(K_CANDIDATES, st, (pm_candidates<kNT, kLanes>), grid_half, dim3(kBlock), 0, st, P, color, planes,
                                A.costs, ca, cplane, cflag, na);
             GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_SELECT_REFINE, st, pm_select_refine<6>, grid_half, dim3(kBlock), 0, st, P, color, it,
+            GMVS_LAUNCH(K_SELECT_REFINE, st, (pm_select_refine<kNT, kLanes>), grid_half, dim3(kBlock), 0, st, P, color, it,
                                planes, A.costs, sel, ca, cplane, cflag, na);
             GMVS_LAUNCH_CHECK();
         }
@@ -1131,7 +1164,7 @@ extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
     if (int rc = setup_views(num_images, cameras, images, nullptr, ws, L, st)) return rc;
     PMConst P = make_const(num_images, cameras, images[0],
                            reinterpret_cast<const ViewDesc*>(ws + L.views), *params, radius);
-    GMVS_LAUNCH(K_COST_BATCH, st, pm_cost_batch<6>, dim3((n + 255) / 256), dim3(256), 0, st, P, n, xy,
+    GMVS_LAUNCH(K_COST_BATCH, st, (pm_cost_batch<kNT, kLanes>), dim3((n * kLanes + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, n, xy,
                        reinterpret_cast<const float4*>(planes), costs);
     GMVS_LAUNCH_CHECK();
     return GMVS_OK;

@@ -29,6 +29,7 @@ if HERE not in sys.path:
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0      # MI355X published HBM3E bandwidth
 FLOPS_PER_NCC = 1150.0     # algorithmic FP32 flops of one 36-tap bilateral NCC
+FLOPS_PER_GEOM = 110.0     # algorithmic FP32 flops of one forward-backward reprojection
 GATHER_BYTES_PER_NCC = 36 * 4 * 4  # 36 taps x 4 bilinear texels x 4 B
 
 
@@ -156,26 +157,32 @@ def main() -> int:
     ms_per_step = elapsed / args.steps * 1e3
     value = ph_step * args.steps / elapsed / 1e6
     log(f"{ms_per_step:.1f} ms/step, {value:.1f} Mpix-hyp/s, fused points {fused}")
-    for k, (cnt, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
-        log(f"  {k:18s} launches {cnt:6d}  total {ms:9.2f} ms  avg {ms / cnt:8.3f} ms")
+    for k, s in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"  {k:18s} launches {s['launches']:6d}  total {s['ms']:9.2f} ms  "
+            f"avg {s['ms'] / s['launches']:8.3f} ms  ncc {s['ncc']:>13d}  geom {s['geom']:>12d}")
 
-    # roofline of the dominant kernel (largest accumulated time)
-    dom = max(stats.items(), key=lambda kv: kv[1][1])
-    dom_name, (dom_cnt, dom_ms) = dom
-    avg_ms = dom_ms / dom_cnt
-    roof = None
-    if dom_name == "pm_candidates":
-        nccs = candidate_ncc_count(h, w, 9)
-        flops = nccs * FLOPS_PER_NCC
-        ach = flops / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                "kernel": dom_name, "avg_ms": round(avg_ms, 4),
-                "ncc_per_launch": nccs,
-                "gather_GBps": round(nccs * GATHER_BYTES_PER_NCC / (avg_ms * 1e-3) / 1e9, 1)}
-    else:
-        roof = {"bound": "valu", "achieved": None, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": None, "traffic": None, "kernel": dom_name, "avg_ms": round(avg_ms, 4)}
+    # roofline of the dominant kernel (largest accumulated time): its
+    # algorithmic FP32 work per launch (counted NCCs and geometric terms x the
+    # per-unit flops) over its average launch duration from the HIP events
+    dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    avg_ms = dom["ms"] / dom["launches"]
+    ncc_per_launch = dom["ncc"] / dom["launches"]
+    geom_per_launch = dom["geom"] / dom["launches"]
+    flops = ncc_per_launch * FLOPS_PER_NCC + geom_per_launch * FLOPS_PER_GEOM
+    ach = flops / (avg_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "kernel": dom_name, "avg_ms": round(avg_ms, 4),
+            "ncc_per_launch": int(ncc_per_launch), "geom_per_launch": int(geom_per_launch),
+            "gather_GBps": round(ncc_per_launch * GATHER_BYTES_PER_NCC / (avg_ms * 1e-3) / 1e9, 1)}
+    if "pm_candidates" in stats:
+        s = stats["pm_candidates"]
+        log(f"pm_candidates NCCs per launch: counted {s['ncc'] / s['launches']:.0f}, "
+            f"analytic {candidate_ncc_count(h, w, 9)}")
+    total_ncc = sum(s["ncc"] for s in stats.values())
+    total_kernel_ms = sum(s["ms"] for s in stats.values())
+    log(f"NCC evaluations per step {total_ncc // args.steps}, "
+        f"{total_ncc / (elapsed) / 1e9:.2f} G NCC/s wall, kernel time {total_kernel_ms / args.steps:.1f} ms/step")
 
     fusion = None
     if not args.no_fusion:

@@ -115,7 +115,8 @@ EXPORTS = (
 
 class CKernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64),
-                ("total_ms", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("ncc", ctypes.c_uint64),
+                ("geom", ctypes.c_uint64)]
 
 
 class GmvsError(RuntimeError):
@@ -182,12 +183,14 @@ def profile_reset() -> None:
 
 
 def profile_collect() -> dict:
-    """{kernel name: (launches, total ms)} accumulated since the last reset."""
+    """{kernel name: dict(launches, ms, ncc, geom)} accumulated since the last
+    reset (ncc / geom: bilateral NCCs and geometric terms evaluated)."""
     n = ctypes.c_int32(0)
     check("gmvs_profile_collect", lib.gmvs_profile_collect(None, 0, ctypes.byref(n)))
     arr = (CKernelStat * max(1, n.value))()
     check("gmvs_profile_collect", lib.gmvs_profile_collect(arr, n.value, ctypes.byref(n)))
-    return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms))
+    return {arr[i].name.decode(): dict(launches=int(arr[i].launches), ms=float(arr[i].total_ms),
+                                       ncc=int(arr[i].ncc), geom=int(arr[i].geom))
             for i in range(n.value)}
 
 

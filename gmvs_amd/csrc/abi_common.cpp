@@ -44,6 +44,7 @@ struct Profiler {
     std::vector<hipEvent_t> pool;
     double ms[K_NUM] = {};
     long long count[K_NUM] = {};
+    unsigned long long* d_work = nullptr;  // [K_NUM][2], device
 
     hipEvent_t take() {
         if (!pool.empty()) {
@@ -67,6 +68,11 @@ thread_local hipEvent_t t_start = nullptr;
 }  // namespace
 
 bool profiling_enabled() { return prof().on; }
+
+unsigned long long* profile_work_counters() {
+    Profiler& p = prof();
+    return p.on ? p.d_work : nullptr;
+}
 
 void profile_begin(hipStream_t st, int kid) {
     Profiler& p = prof();
@@ -96,8 +102,14 @@ void profile_end(hipStream_t st, int kid) {
 }  // namespace gmvs
 
 extern "C" int gmvs_profile_enable(int32_t on) {
+    gmvs::clear_error();
     gmvs::Profiler& p = gmvs::prof();
     std::lock_guard<std::mutex> lk(p.mu);
+    if (on && !p.d_work) {
+        const size_t bytes = sizeof(unsigned long long) * 2 * gmvs::K_NUM;
+        GMVS_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p.d_work), bytes));
+        GMVS_HIP_CHECK(hipMemset(p.d_work, 0, bytes));
+    }
     p.on = on != 0;
     return GMVS_OK;
 }
@@ -116,6 +128,9 @@ extern "C" int gmvs_profile_collect(gmvs_kernel_stat* out, int32_t max_out, int3
         p.pool.push_back(q.b);
     }
     p.pending.clear();
+    unsigned long long work[2 * gmvs::K_NUM] = {};
+    if (p.d_work)
+        GMVS_HIP_CHECK(hipMemcpy(work, p.d_work, sizeof(work), hipMemcpyDeviceToHost));
     int n = 0;
     for (int k = 0; k < gmvs::K_NUM; ++k) {
         if (p.count[k] == 0) continue;
@@ -123,6 +138,8 @@ extern "C" int gmvs_profile_collect(gmvs_kernel_stat* out, int32_t max_out, int3
             std::snprintf(out[n].name, sizeof(out[n].name), "%s", gmvs::kKernelNames[k]);
             out[n].launches = p.count[k];
             out[n].total_ms = p.ms[k];
+            out[n].ncc = work[2 * k];
+            out[n].geom = work[2 * k + 1];
         }
         ++n;
     }
@@ -131,6 +148,7 @@ extern "C" int gmvs_profile_collect(gmvs_kernel_stat* out, int32_t max_out, int3
 }
 
 extern "C" int gmvs_profile_reset(void) {
+    gmvs::clear_error();
     gmvs::Profiler& p = gmvs::prof();
     std::lock_guard<std::mutex> lk(p.mu);
     for (const gmvs::Pending& q : p.pending) {
@@ -142,6 +160,10 @@ extern "C" int gmvs_profile_reset(void) {
     for (int k = 0; k < gmvs::K_NUM; ++k) {
         p.ms[k] = 0.0;
         p.count[k] = 0;
+    }
+    if (p.d_work) {
+        GMVS_HIP_CHECK(hipDeviceSynchronize());
+        GMVS_HIP_CHECK(hipMemset(p.d_work, 0, sizeof(unsigned long long) * 2 * gmvs::K_NUM));
     }
     return GMVS_OK;
 }

@@ -37,6 +37,9 @@ enum KernelId {
 bool profiling_enabled();
 void profile_begin(hipStream_t st, int kid);
 void profile_end(hipStream_t st, int kid);
+// Device counters [K_NUM][2] (NCCs, geometric terms) while profiling is on,
+// else nullptr.
+unsigned long long* profile_work_counters();
 
 struct ProfScope {
     hipStream_t st;

@@ -372,9 +372,9 @@ template <int NT, int L>
 __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9],
                                  const float4* __restrict__ quad_ptr, int h, int w) {
     using B = TapBlock<NT, L>;
-    const float cz = fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8]));
-    const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) / cz;
-    const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) / cz;
+    const float rcz = __builtin_amdgcn_rcpf(fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8])));
+    const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
+    const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
     if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
     gfloat4* quad = as_global(quad_ptr);
     const float wm1 = (float)(w - 1), hm1 = (float)(h - 1);
@@ -420,7 +420,7 @@ __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9],
     const float var_s = S2 * hs.inv_w - ms * ms;
     if (hs.var_r < 1e-5f || var_s < 1e-5f) return 2.0f;
     const float cov = X * hs.inv_w - hs.mean_r * ms;
-    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / sqrtf(hs.var_r * var_s)));
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov * __builtin_amdgcn_rsqf(hs.var_r * var_s)));
 }
 
 }  // namespace gmvs

@@ -5,7 +5,8 @@
 // gmvs/lib/src/patch_match_kernel.cu:1579-1653) runs as two kernels:
 //   pm_candidates     -- adaptive candidate search over the state as it was at
 //                        the start of the half-sweep, plus the 8 x V NCC cost
-//                        table the view selection needs. It writes nothing
+//                        table the view selection needs and the V NCCs of
+//                        the pixel's current hypothesis. It writes nothing
 //                        to the state, only a per-pixel record (costs, the
 //                        candidate planes, the flags) into the workspace;
 //   pm_select_refine  -- joint view selection, candidate adoption (plain or
@@ -62,7 +63,23 @@ struct PMConst {
     int top_k;
     uint32_t seed, view;
     int init_mode, geom, prior;
+    unsigned long long* work;  // profiling work counters [K_NUM][2] or nullptr
 };
+
+// Adds per-lane work counts (NCCs, geometric terms) to a kernel's counters:
+// one reduction across the wave and one atomic per wave. Every lane of the
+// wave must be active.
+__device__ inline void count_work(unsigned long long* ctr, uint32_t ncc, uint32_t geom) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ncc += __shfl_xor(ncc, off);
+        geom += __shfl_xor(geom, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (ncc) atomicAdd(ctr, (unsigned long long)ncc);
+        if (geom) atomicAdd(ctr + 1, (unsigned long long)geom);
+    }
+}
 
 // Block -> tile with the tiles of one XCD contiguous, so that neighbouring
 // tiles (which gather from the same source-image region) share an L2.
@@ -145,11 +162,10 @@ __global__ void pm_pack_quads(const float* __restrict__ img, float4* __restrict_
 
 // ---------------------------------------------------------------- init ---
 template <int NT, int L>
-__global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict__ planes,
-                                                  float* __restrict__ costs,
-                                                  uint32_t* __restrict__ sel) {
+__device__ inline void pm_init_body(const PMConst& P, float4* __restrict__ planes,
+                                    float* __restrict__ costs, uint32_t* __restrict__ sel,
+                                    float (*s_c)[kBlock], uint32_t& ncc) {
     using TL = Tiling<L>;
-    __shared__ float s_c[kMaxViews][kBlock];
     int tile;
     if (!tile_of_block(TL::tiles(P.w, P.h), tile)) return;
     const int tx = (P.w + TL::TX - 1) / TL::TX;
@@ -247,7 +263,18 @@ __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict_
         planes[c] = pl;
         costs[c] = cost;
         sel[c] = bits;
+        ncc = (uint32_t)P.nv;
     }
+}
+
+template <int NT, int L>
+__global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict__ planes,
+                                                  float* __restrict__ costs,
+                                                  uint32_t* __restrict__ sel) {
+    __shared__ float s_c[kMaxViews][kBlock];
+    uint32_t ncc = 0;
+    pm_init_body<NT, L>(P, planes, costs, sel, s_c, ncc);
+    if (P.work) count_work(P.work + 2 * K_INIT, ncc, 0);
 }
 
 // ------------------------------------------------------- half-sweep, A ---
@@ -444,12 +471,11 @@ __device__ inline void candidate_search(const float* __restrict__ cs, int px, in
 }
 
 template <int NT, int L>
-__global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
-                                                        const float4* __restrict__ planes,
-                                                        const float* __restrict__ costs,
-                                                        float* __restrict__ ca,
-                                                        float4* __restrict__ cplane,
-                                                        uint8_t* __restrict__ cflag, int NA) {
+__device__ inline void pm_candidates_body(const PMConst& P, int color,
+                                          const float4* __restrict__ planes,
+                                          const float* __restrict__ costs, float* __restrict__ ca,
+                                          float4* __restrict__ cplane, uint8_t* __restrict__ cflag,
+                                          int NA, uint32_t& ncc) {
     int tile;
     if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
     const Active a = active_pixel<L>(P, tile, color);
@@ -458,7 +484,10 @@ __global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
     uint32_t flags;
     int pos[8];
     candidate_search(costs, a.px, a.py, P.h, P.w, flags, pos);
-    if (lead) cflag[a.aidx] = (uint8_t)flags;
+    if (lead) {
+        cflag[a.aidx] = (uint8_t)flags;
+        ncc = (uint32_t)((__popc(flags) + 1) * P.nv);
+    }
     Hoist<NT, L> hs;
     hoist_ref<NT, L>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub);
     for (int k = 0; k < 8; ++k) {
@@ -475,6 +504,32 @@ __global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
             if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
     }
+    // row 8: the pixel's own current hypothesis against every view (the
+    // select/refine kernel weights the selected ones)
+    {
+        const float4 pl = planes[a.py * P.w + a.px];
+        float m[3];
+        plane_m(P.rc, pl, m);
+        for (int v = 0; v < P.nv; ++v) {
+            const ViewDesc& vd = P.views[v];
+            float H[9];
+            homography(vd.A, vd.b, m, H);
+            const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+            if (lead) ca[((size_t)8 * P.nv + v) * NA + a.aidx] = cv;
+        }
+    }
+}
+
+template <int NT, int L>
+__global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
+                                                        const float4* __restrict__ planes,
+                                                        const float* __restrict__ costs,
+                                                        float* __restrict__ ca,
+                                                        float4* __restrict__ cplane,
+                                                        uint8_t* __restrict__ cflag, int NA) {
+    uint32_t ncc = 0;
+    pm_candidates_body<NT, L>(P, color, planes, costs, ca, cplane, cflag, NA, ncc);
+    if (P.work) count_work(P.work + 2 * K_CANDIDATES, ncc, 0);
 }
 
 // ------------------------------------------------------- half-sweep, B ---
@@ -529,19 +584,11 @@ __device__ inline float ca_entry(const float* __restrict__ ca, uint32_t flags, i
 }
 
 template <int NT, int L>
-__global__ __launch_bounds__(kBlock) void pm_select_refine(
-    PMConst P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
+__device__ inline void pm_select_refine_body(
+    const PMConst& P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
     uint32_t* __restrict__ sel, const float* __restrict__ ca, const float4* __restrict__ cplane,
-    const uint8_t* __restrict__ cflag, int NA) {
-    __shared__ float s_cdf[kMaxViews][kBlock];
-    __shared__ ViewDesc s_views[kMaxViews];
-    {
-        const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
-        for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
-        __syncthreads();
-    }
+    const uint8_t* __restrict__ cflag, int NA, float (*s_cdf)[kBlock], const ViewDesc* s_views,
+    uint32_t& ncc_count, uint32_t& geom_count) {
     int tile;
     if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
     const Active a = active_pixel<L>(P, tile, color);
@@ -680,8 +727,24 @@ __global__ __launch_bounds__(kBlock) void pm_select_refine(
         return s / wsum;
     };
 
+    // The current hypothesis re-costed with the new view weights; its NCCs
+    // come from pm_candidates (row 8 of the record), views with zero weight
+    // add exactly zero and are skipped.
     float4 cur = planes[c];
-    const float cnow = wcost(cur);
+    float cnow;
+    {
+        float s = 0.f;
+        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+            const int v = __ffs(mm) - 1;
+            const float x = ca[((size_t)8 * nv + v) * NA + aidx];
+            const float wv = weight_of(v);
+            if (P.geom)
+                s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cur, fpx, fpy));
+            else
+                s += wv * x;
+        }
+        cnow = s / wsum;
+    }
     float cost = cnow;
     uint32_t selc = sel[c];
     float depth_now = depth_from_plane(rc, cur, fpx, fpy);
@@ -777,7 +840,30 @@ __global__ __launch_bounds__(kBlock) void pm_select_refine(
         planes[c] = cur;
         costs[c] = cost;
         sel[c] = selc;
+        const uint32_t ns = (uint32_t)__popc(selmask);
+        ncc_count = 5u * ns;  // the 5 refinement candidates
+        geom_count = P.geom ? ((uint32_t)__popc(flags) + 6u) * ns : 0u;
     }
+}
+
+template <int NT, int L>
+__global__ __launch_bounds__(kBlock) void pm_select_refine(
+    PMConst P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
+    uint32_t* __restrict__ sel, const float* __restrict__ ca, const float4* __restrict__ cplane,
+    const uint8_t* __restrict__ cflag, int NA) {
+    __shared__ float s_cdf[kMaxViews][kBlock];
+    __shared__ ViewDesc s_views[kMaxViews];
+    {
+        const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
+        for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+        __syncthreads();
+    }
+    uint32_t ncc = 0, geom = 0;
+    pm_select_refine_body<NT, L>(P, color, iter, planes, costs, sel, ca, cplane, cflag, NA, s_cdf,
+                                 s_views, ncc, geom);
+    if (P.work) count_work(P.work + 2 * K_SELECT_REFINE, ncc, geom);
 }
 
 // ------------------------------------------------------ finalisation ---
@@ -935,7 +1021,7 @@ PMLayout pm_layout(int h, int w, int num_images, bool sweep) {
     if (sweep) {
         o = align_up(o + sizeof(uint32_t) * hw);
         L.ca = o;
-        o = align_up(o + sizeof(float) * 8 * nv * na);
+        o = align_up(o + sizeof(float) * 9 * nv * na);
         L.cplane = o;
         o = align_up(o + sizeof(float4) * 8 * na);
         L.cflag = o;
@@ -1091,6 +1177,7 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
     P.init_mode = A.init_mode;
     P.geom = A.geom ? 1 : 0;
     P.prior = A.params.planar_prior ? 1 : 0;
+    P.work = profile_work_counters();
 
     float4* planes = reinterpret_cast<float4*>(A.planes);
     uint32_t* sel = reinterpret_cast<uint32_t*>(ws + L.sel);

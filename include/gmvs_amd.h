@@ -185,11 +185,15 @@ int gmvs_delaunay(const int32_t* xy, int32_t n, int32_t* tri_out, int32_t max_tr
  * every kernel launch of the library is bracketed by hipEvents on its
  * stream; gmvs_profile_collect waits for the recorded events and returns the
  * accumulated launches and milliseconds per kernel name (out may be NULL to
- * query the count). */
+ * query the count). The PatchMatch kernels also count the bilateral NCCs
+ * (one reference pixel, one plane, one source view) and geometric-term
+ * evaluations they perform. */
 typedef struct gmvs_kernel_stat {
     char name[32];
     int64_t launches;
     double total_ms;
+    uint64_t ncc;
+    uint64_t geom;
 } gmvs_kernel_stat;
 int gmvs_profile_enable(int32_t on);
 int gmvs_profile_collect(gmvs_kernel_stat* out, int32_t max_out, int32_t* num_out);

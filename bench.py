@@ -111,6 +111,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the workload, run one device-to-device copy of this many MiB "
+                         "(the known-size reference for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
@@ -124,6 +127,15 @@ def main() -> int:
     from gmvs_amd.pipeline import MultiViewPipeline, cameras_from_synthetic, fuse_maps, pixel_hypotheses
 
     dev = "cuda"
+    if args.copy_probe_mib > 0:
+        nel = args.copy_probe_mib * (1 << 18)
+        src = torch.ones(nel, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {nel * 4} bytes read, {nel * 4} bytes written")
+        del src, dst
     log("generating the 10-view 1600x1200 scene ...")
     sc, labels = make_workload(dev)
     cams = cameras_from_synthetic(sc.cameras)

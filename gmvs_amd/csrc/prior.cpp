@@ -25,26 +25,87 @@
 namespace gmvs {
 namespace {
 
+// A vertex (x + sx*M, y + sy*M). Input points have sx = sy = 0; the three
+// super-triangle vertices are at infinity (M -> infinity): their predicates
+// are evaluated as polynomials in M and take the sign of the leading nonzero
+// coefficient. With the bounding triangle infinitely far away, every triangle
+// of input points whose circumcircle is empty survives, including the thin
+// triangles along the convex hull that a finite bounding triangle loses.
 struct Pt {
     int64_t x, y;
+    int sx = 0, sy = 0;
+    bool real() const { return sx == 0 && sy == 0; }
 };
 
-inline int64_t orient(const Pt& a, const Pt& b, const Pt& c) {
-    return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+typedef __int128 i128;
+
+// c[0] + c[1]*M + ... ; the sign of the polynomial as M -> infinity.
+template <int N>
+inline int limit_sign(const i128 (&c)[N]) {
+    for (int i = N - 1; i >= 0; --i)
+        if (c[i] != 0) return c[i] > 0 ? 1 : -1;
+    return 0;
 }
 
-// > 0 when d lies strictly inside the circumcircle of the counter-clockwise
-// triangle (a, b, c).
+struct Lin {  // c0 + c1*M
+    i128 c0, c1;
+};
+inline Lin sub(const Pt& a, const Pt& b) { return Lin{a.x - b.x, i128(a.sx - b.sx)}; }
+inline void mul(const Lin& a, const Lin& b, i128 (&o)[3]) {
+    o[0] = a.c0 * b.c0;
+    o[1] = a.c0 * b.c1 + a.c1 * b.c0;
+    o[2] = a.c1 * b.c1;
+}
+inline Lin suby(const Pt& a, const Pt& b) { return Lin{a.y - b.y, i128(a.sy - b.sy)}; }
+
+// sign of (b - a) x (c - a): > 0 for a counter-clockwise triangle in x-right,
+// y-up axes.
+inline int orient(const Pt& a, const Pt& b, const Pt& c) {
+    if (a.real() && b.real() && c.real()) {
+        const int64_t v = (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+        return (v > 0) - (v < 0);
+    }
+    i128 p[3], q[3];
+    mul(sub(b, a), suby(c, a), p);
+    mul(suby(b, a), sub(c, a), q);
+    const i128 r[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+    return limit_sign(r);
+}
+
+// true when the input point d lies strictly inside the circumcircle of the
+// positively oriented triangle (a, b, c).
 inline bool in_circle(const Pt& a, const Pt& b, const Pt& c, const Pt& d) {
-    const __int128 adx = a.x - d.x, ady = a.y - d.y;
-    const __int128 bdx = b.x - d.x, bdy = b.y - d.y;
-    const __int128 cdx = c.x - d.x, cdy = c.y - d.y;
-    const __int128 ad = adx * adx + ady * ady;
-    const __int128 bd = bdx * bdx + bdy * bdy;
-    const __int128 cd = cdx * cdx + cdy * cdy;
-    const __int128 det = ad * (bdx * cdy - bdy * cdx) - bd * (adx * cdy - ady * cdx) +
+    if (a.real() && b.real() && c.real()) {
+        const i128 adx = a.x - d.x, ady = a.y - d.y;
+        const i128 bdx = b.x - d.x, bdy = b.y - d.y;
+        const i128 cdx = c.x - d.x, cdy = c.y - d.y;
+        const i128 ad = adx * adx + ady * ady;
+        const i128 bd = bdx * bdx + bdy * bdy;
+        const i128 cd = cdx * cdx + cdy * cdy;
+        const i128 det = ad * (bdx * cdy - bdy * cdx) - bd * (adx * cdy - ady * cdx) +
                          cd * (adx * bdy - ady * bdx);
-    return det > 0;
+        return det > 0;
+    }
+    const Lin X[3] = {sub(a, d), sub(b, d), sub(c, d)};
+    const Lin Y[3] = {suby(a, d), suby(b, d), suby(c, d)};
+    i128 lift[3][3], cross[3][3];  // lift[k] = |k - d|^2; cross[k] = the 2x2 minor without k
+    for (int k = 0; k < 3; ++k) {
+        i128 xx[3], yy[3];
+        mul(X[k], X[k], xx);
+        mul(Y[k], Y[k], yy);
+        for (int j = 0; j < 3; ++j) lift[k][j] = xx[j] + yy[j];
+        const int u = (k + 1) % 3, v = (k + 2) % 3;
+        i128 p[3], q[3];
+        mul(X[u], Y[v], p);
+        mul(Y[u], X[v], q);
+        for (int j = 0; j < 3; ++j) cross[k][j] = p[j] - q[j];
+    }
+    // det = sum_k lift[k] * cross[k] (cyclic minors carry the cofactor signs)
+    i128 det[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) det[i + j] += lift[k][i] * cross[k][j];
+    return limit_sign(det) > 0;
 }
 
 // Hilbert index of (x, y) on a 2^16 grid (the standard xy -> d
@@ -76,13 +137,10 @@ struct Tri {
 class Delaunay {
 public:
     explicit Delaunay(const std::vector<Pt>& pts) : p_(pts) {
-        int64_t mx = 1;
-        for (const Pt& q : pts) mx = std::max<int64_t>(mx, std::max<int64_t>(q.x < 0 ? -q.x : q.x, q.y < 0 ? -q.y : q.y));
-        const int64_t M = 64 * mx + 1024;  // super-triangle far outside the points
         n_ = int(pts.size());
-        p_.push_back(Pt{-M, -M});
-        p_.push_back(Pt{M, -M});
-        p_.push_back(Pt{0, M});
+        p_.push_back(Pt{0, 0, -1, -1});  // the bounding triangle, at infinity
+        p_.push_back(Pt{0, 0, 1, -1});
+        p_.push_back(Pt{0, 0, 0, 1});
         Tri t;
         t.v[0] = n_;
         t.v[1] = n_ + 1;
@@ -178,7 +236,9 @@ private:
         const int t0 = locate(q);
         if (t0 < 0) return;
         for (int k = 0; k < 3; ++k)
-            if (p_[tris_[t0].v[k]].x == q.x && p_[tris_[t0].v[k]].y == q.y) return;  // duplicate
+            if (p_[tris_[t0].v[k]].real() && p_[tris_[t0].v[k]].x == q.x &&
+                p_[tris_[t0].v[k]].y == q.y)
+                return;  // duplicate
         // cavity: triangles whose circumcircle contains q, grown from t0
         std::vector<int> bad{t0};
         std::vector<int> stack{t0};

@@ -52,6 +52,21 @@ def build_prior(ref_cam: N.CCamera, depth: np.ndarray, cost: np.ndarray,
     return planes, masks.astype(np.int32), int(ntri.value)
 
 
+def delaunay(xy: np.ndarray) -> np.ndarray:
+    """Delaunay triangulation of integer points (gmvs_delaunay; the
+    triangulation step of planar_initer.cu:347-361). xy: [n,2] ints in
+    [0, 65536), distinct. Returns [T,3] int32 vertex indices, each triangle
+    with (b - a) x (c - a) > 0 in pixel coordinates (x right, y down)."""
+    pts = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
+    ntri = ctypes.c_int32(0)
+    N.check("gmvs_delaunay", N.lib.gmvs_delaunay(pts.ctypes.data, len(pts), None, 0,
+                                                 ctypes.byref(ntri)))
+    tri = np.zeros((max(1, ntri.value), 3), np.int32)
+    N.check("gmvs_delaunay", N.lib.gmvs_delaunay(pts.ctypes.data, len(pts), tri.ctypes.data,
+                                                 ntri.value, ctypes.byref(ntri)))
+    return tri[:ntri.value]
+
+
 class PlanarInit:
     """PlanarInit (gmvs/lib/include/planar_initer.cuh:65-133)."""
 

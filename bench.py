@@ -104,6 +104,68 @@ def fusion_workload(device: str):
     return sc, d.contiguous(), sc.normal.contiguous(), pairs
 
 
+def _short_kernel(name: str) -> str:
+    name = name.strip().strip('"')
+    if name.startswith("void "):
+        name = name[5:]
+    depth = 0
+    for i, ch in enumerate(name):  # cut the argument list, keep template args
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            name = name[:i]
+            break
+    return name[:90]
+
+
+def summarize_profile(prof_dir: str) -> str:
+    """Compact text summary of rocprofv3 CSV output in prof_dir: the
+    kernel-stats table and, for every counter-collection CSV, the mean value
+    per dispatch of each counter for each kernel."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    out = []
+    for path in sorted(glob.glob(os.path.join(prof_dir, "**", "*kernel_stats.csv"), recursive=True)):
+        out.append(f"== {os.path.basename(path)} (top 20 by total time)")
+        with open(path, newline="") as f:
+            rows = list(csv.DictReader(f))
+        rows.sort(key=lambda r: -float(r.get("TotalDurationNs", 0) or 0))
+        out.append(f"{'kernel':60s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>10s} {'min_us':>9s} {'max_us':>9s}")
+        for r in rows[:20]:
+            out.append(f"{_short_kernel(r['Name'])[:60]:60s} {int(r['Calls']):7d} "
+                       f"{float(r['TotalDurationNs']) / 1e6:10.2f} {float(r['AverageNs']) / 1e3:10.1f} "
+                       f"{float(r['MinNs']) / 1e3:9.1f} {float(r['MaxNs']) / 1e3:9.1f}")
+    for path in sorted(glob.glob(os.path.join(prof_dir, "**", "*counter_collection.csv"),
+                                       recursive=True)):
+        acc = defaultdict(float)
+        disp = defaultdict(set)
+        top = []  # the largest single dispatches per counter (finds the copy probe)
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                k = _short_kernel(r["Kernel_Name"])
+                v = float(r["Counter_Value"])
+                acc[(k, r["Counter_Name"])] += v
+                disp[k].add(r["Dispatch_Id"])
+                if "copy" in k.lower():
+                    top.append((v, r["Counter_Name"], k, r["Dispatch_Id"], r.get("Grid_Size", "")))
+                    top = sorted(top, reverse=True)[:4]
+        counters = sorted({c for _, c in acc})
+        out.append(f"== {os.path.basename(path)}: mean per dispatch")
+        out.append(f"{'kernel':60s} {'disp':>6s} " + " ".join(f"{c:>20s}" for c in counters))
+        kern = sorted(disp, key=lambda k: -sum(acc[(k, c)] for c in counters))
+        for k in kern[:25]:
+            n = len(disp[k])
+            out.append(f"{k[:60]:60s} {n:6d} " + " ".join(f"{acc[(k, c)] / n:20.1f}" for c in counters))
+        out.append("largest copy-kernel dispatches: value, counter, kernel, dispatch id, grid size")
+        for t in top:
+            out.append(f"  {t[0]:.1f} {t[1]} {t[2]} {t[3]} {t[4]}")
+    return "\n".join(out)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,10 +173,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
+    ap.add_argument("--summarize-profile", metavar="DIR", default=None,
+                    help="print a summary of rocprofv3 CSV output in DIR and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the workload, run one device-to-device copy of this many MiB "
                          "(the known-size reference for hardware byte counters)")
     args = ap.parse_args()
+    if args.summarize_profile:
+        print(summarize_profile(args.summarize_profile))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not run)"}))

@@ -1,9 +1,10 @@
 """Builds libgmvs_amd.so (HIP kernels for gfx950 + the C ABI) in-tree.
 
 Every translation unit is compiled with hipcc for gfx950 only; packed FP32
-instructions are disabled (-packed-fp32-ops) and FP32 division/sqrt are
-correctly rounded. fusion.hip additionally disables FMA contraction so that
-its arithmetic matches a sequential CPU loop bit for bit.
+instructions are disabled (-packed-fp32-ops). FP32 division/sqrt are
+correctly rounded except in pm_kernels.hip, whose results are checked against
+the FP64 oracle within a tolerance. fusion.hip additionally disables FMA
+contraction so that its arithmetic matches a sequential CPU loop bit for bit.
 """
 from __future__ import annotations
 
@@ -37,7 +38,9 @@ COMMON = [
 SOURCES = {
     "abi_common.cpp": [],
     "prior.cpp": [],
-    "pm_kernels.hip": [],
+    # PatchMatch: FP32 division and sqrt as rcp/sqrt instructions (~1-2.5 ulp);
+    # costs are compared against the FP64 oracle within a tolerance
+    "pm_kernels.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "fusion.hip": ["-ffp-contract=off"],
 }
 

@@ -171,6 +171,19 @@ __device__ inline float4 random_normal(const RefCam& c, float px, float py, cons
     return n;
 }
 
+// e^x through the hardware exp2 (v_exp_f32, ~1 ulp in the exp2 domain).
+// Used for the bilateral weights, the view-selection likelihoods and the
+// prior scores, where FP32 rounding only moves near-ties.
+__device__ inline float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
+// sin / cos of a small angle (|a| < pi) through v_sin_f32 / v_cos_f32, whose
+// argument is in revolutions.
+__device__ inline void fsincos(float a, float* s, float* c) {
+    const float r = a * 0.159154943091895336f;
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
+
 // Euler-angle perturbation of a normal (A17) from three fixed slots.
 __device__ inline float4 perturbed_normal(const RefCam& c, float px, float py, float4 n,
                                           float u1, float u2, float u3, float pert) {
@@ -178,9 +191,9 @@ __device__ inline float4 perturbed_normal(const RefCam& c, float px, float py, f
     view_dir(c, px, py, 1.0f, vd);
     const float a1 = (u1 - 0.5f) * pert, a2 = (u2 - 0.5f) * pert, a3 = (u3 - 0.5f) * pert;
     float s1, c1, s2, c2, s3, c3;
-    sincosf(a1, &s1, &c1);
-    sincosf(a2, &s2, &c2);
-    sincosf(a3, &s3, &c3);
+    fsincos(a1, &s1, &c1);
+    fsincos(a2, &s2, &c2);
+    fsincos(a3, &s3, &c3);
     const float R0 = c2 * c3, R1 = c3 * s1 * s2 - c1 * s3, R2 = s1 * s3 + c1 * c3 * s2;
     const float R3 = c2 * s3, R4 = c1 * c3 + s1 * s2 * s3, R5 = c1 * s2 * s3 - c3 * s1;
     const float R6 = -s2, R7 = c2 * s1, R8 = c1 * c2;
@@ -338,7 +351,7 @@ __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref
         for (int q = 0; q < B::NC; ++q) {
             const float dj = (float)(-radius + B::col(sub, q) * inc);
             const float rv = ref[(int)hs.ty[q] * w + (int)hs.tx[r]] - c;
-            const float wgt = expf(-sqrtf(di * di + dj * dj) / ss - fabsf(rv) / sc);
+            const float wgt = fexp(-sqrtf(di * di + dj * dj) / ss - fabsf(rv) / sc);
             hs.wt[r * B::NC + q] = wgt;
             hs.wr[r * B::NC + q] = wgt * rv;
             W += wgt;

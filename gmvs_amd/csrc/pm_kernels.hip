@@ -571,8 +571,8 @@ __device__ inline float prior_score(const PMConst& P, float cost, float depth, f
     float dot = pp.x * pl.x + pp.y * pl.y + pp.z * pl.z;
     dot = dot > 1.0f ? 1.0f : (dot < -1.0f ? -1.0f : dot);
     const float ang = acosf(dot);
-    const float pr = 0.5f + expf(-dd * dd / two_ds2) * expf(-ang * ang / two_as2);
-    return expf(-cost * cost / 0.18f) * pr;
+    const float pr = 0.5f + fexp(-dd * dd / two_ds2) * fexp(-ang * ang / two_as2);
+    return fexp(-cost * cost / 0.18f) * pr;
 }
 
 __device__ inline float ca_entry(const float* __restrict__ ca, uint32_t flags, int k, int v, int nv,
@@ -606,7 +606,7 @@ __device__ inline void pm_select_refine_body(
     const int nb[4] = {c - P.w, c + P.w, c - 1, c + 1};
 #pragma unroll
     for (int i = 0; i < 4; ++i) nbsel[i] = ((flags >> (2 * i)) & 1u) ? sel[nb[i]] : 0u;
-    const float tau = 0.8f * expf((float)(iter * iter) / -90.0f);
+    const float tau = 0.8f * fexp((float)(iter * iter) / -90.0f);
     float psum = 0.f;
     for (int v = 0; v < nv; ++v) {
         float pri = 0.f;
@@ -619,7 +619,7 @@ __device__ inline void pm_select_refine_body(
         for (int k = 0; k < 8; ++k) {
             const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
             if (x < tau) {
-                tmpw += expf(x * x / -0.18f);
+                tmpw += fexp(x * x / -0.18f);
                 count += 1.0f;
             }
             cf += (x > 1.2f);
@@ -628,7 +628,7 @@ __device__ inline void pm_select_refine_body(
         if (count > 2.0f && cf < 3)
             p = tmpw / count;
         else if (cf < 3)
-            p = expf(tau * tau / -0.32f);
+            p = fexp(tau * tau / -0.32f);
         p = p * pri;
         s_cdf[v][threadIdx.x] = p;
         psum += p;

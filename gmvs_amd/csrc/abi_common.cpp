@@ -28,7 +28,7 @@ void clear_error() { g_error.clear(); }
 namespace {
 
 const char* const kKernelNames[K_NUM] = {
-    "pm_setup_view", "pm_pack_quads", "pm_init",      "pm_candidates",  "pm_select_refine",
+    "pm_setup_view", "pm_pack_quads", "pm_init",      "pm_candidates",  "pm_select",  "pm_refine",
     "pm_depth_normal", "pm_filter",   "pm_export",    "pm_cost_batch",  "fuse_fill",
     "fuse_check",     "fuse_scatter", "fuse_write_view"};
 

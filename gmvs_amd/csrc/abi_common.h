@@ -22,7 +22,8 @@ enum KernelId {
     K_PACK,
     K_INIT,
     K_CANDIDATES,
-    K_SELECT_REFINE,
+    K_SELECT,
+    K_REFINE,
     K_DEPTH_NORMAL,
     K_FILTER,
     K_EXPORT,

@@ -2,20 +2,23 @@
 //
 // One half-sweep of the red/black checkerboard (black_red_pixel_update,
 // gmvs/src/patch_match.cu:1085-1142; Black/RedPixelUpdate,
-// gmvs/lib/src/patch_match_kernel.cu:1579-1653) runs as two kernels:
-//   pm_candidates     -- adaptive candidate search over the state as it was at
-//                        the start of the half-sweep, plus the 8 x V NCC cost
-//                        table the view selection needs and the V NCCs of
-//                        the pixel's current hypothesis. It writes nothing
-//                        to the state, only a per-pixel record (costs, the
-//                        candidate planes, the flags) into the workspace;
-//   pm_select_refine  -- joint view selection, candidate adoption (plain or
-//                        planar-prior scoring, optional geometric term) and
-//                        random refinement, costed over the selected views
-//                        only; each thread writes its own pixel.
-// Because pm_select_refine reads neighbour planes only from the records,
-// every read of the half-sweep sees the snapshot (the reference reads
-// same-colour pixels that other threads are rewriting, lib 978-1104).
+// gmvs/lib/src/patch_match_kernel.cu:1579-1653) runs as three kernels:
+//   pm_candidates -- adaptive candidate search over the state as it was at
+//                    the start of the half-sweep, plus the 8 x V NCC cost
+//                    table the view selection needs and the V NCCs of the
+//                    pixel's current hypothesis. It writes nothing to the
+//                    state, only a per-pixel record (costs, the candidate
+//                    planes, the flags) into the workspace;
+//   pm_select     -- joint view selection and candidate adoption (plain or
+//                    planar-prior scoring, optional geometric term), one
+//                    thread per pixel; writes the pixel's adopted state and
+//                    its 5 random refinement hypotheses;
+//   pm_refine     -- the refinement hypotheses costed over the selected views
+//                    only, as load-balanced work units, and adopted.
+// Each thread writes only its own pixel, and neighbour planes are read only
+// from the candidate records, so every read of the half-sweep sees the
+// snapshot (the reference reads same-colour pixels that other threads are
+// rewriting, lib 978-1104).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -533,6 +536,22 @@ __global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
 }
 
 // ------------------------------------------------------- half-sweep, B ---
+// The second half of a half-sweep runs as two kernels:
+//   pm_select  -- one thread per active pixel: joint view selection
+//                 (src 915-981), candidate adoption with plain or
+//                 planar-prior scoring and the optional geometric term
+//                 (src 983-1040, lib 1185-1334); writes the adopted state
+//                 and the 5 refinement hypotheses (src 595-676,
+//                 lib 684-811) with the view weights to the workspace;
+//   pm_refine  -- one 8x8 tile of active pixels per block: the refinement
+//                 NCCs as work units (pixel, selected view), each costing the
+//                 5 hypotheses against one view, dealt round-robin over the
+//                 block's 64 lane groups so that trip counts differ by at most
+//                 one whatever the per-pixel view counts are; then the
+//                 reference's sequential adoption in view order.
+constexpr int kK2Pix = kBlock / kLanes;  // active pixels per pm_refine block
+constexpr int kRefine = 5;               // refinement hypotheses per pixel
+
 __device__ inline int argmin_last8(const float* c) {
     float m = c[0];
     int k = 0;
@@ -583,287 +602,471 @@ __device__ inline float ca_entry(const float* __restrict__ ca, uint32_t flags, i
     return (k == 0 && v == 0) ? 2.0f : 0.0f;
 }
 
-template <int NT, int L>
-__device__ inline void pm_select_refine_body(
-    const PMConst& P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
-    uint32_t* __restrict__ sel, const float* __restrict__ ca, const float4* __restrict__ cplane,
-    const uint8_t* __restrict__ cflag, int NA, float (*s_cdf)[kBlock], const ViewDesc* s_views,
-    uint32_t& ncc_count, uint32_t& geom_count) {
-    int tile;
-    if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
-    const Active a = active_pixel<L>(P, tile, color);
-    if (!a.ok) return;
-    const int px = a.px, py = a.py, aidx = a.aidx;
-    const int c = py * P.w + px;
-    const float fpx = (float)px, fpy = (float)py;
-    const RefCam& rc = P.rc;
-    const int nv = P.nv;
-    const uint32_t flags = cflag[aidx];
-    const Rng rng{(uint32_t)c, (uint32_t)iter, P.seed, P.view};
+// Per-pixel record handed from pm_select to pm_refine (workspace, [word][NA]).
+enum SelWord {
+    SW_SELMASK = 0,  // selected views
+    SW_WNIB0,        // 4-bit draw counts, views 0-7 .. 24-31
+    SW_WNIB1,
+    SW_WNIB2,
+    SW_WNIB3,
+    SW_RESTRICTED,   // prior score of the adopted hypothesis (float bits)
+    SW_DRAND,        // refinement depths (float bits)
+    SW_DPERT,
+    SW_DCUR,
+    SW_NUM
+};
 
-    // ---- joint view selection (src 915-981)
-    uint32_t nbsel[4];
-    const int nb[4] = {c - P.w, c + P.w, c - 1, c + 1};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) nbsel[i] = ((flags >> (2 * i)) & 1u) ? sel[nb[i]] : 0u;
-    const float tau = 0.8f * fexp((float)(iter * iter) / -90.0f);
-    float psum = 0.f;
-    for (int v = 0; v < nv; ++v) {
-        float pri = 0.f;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if ((flags >> (2 * i)) & 1u) pri += ((nbsel[i] >> v) & 1u) ? 0.9f : 0.1f;
-        float count = 0.f, tmpw = 0.f;
-        int cf = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
-            if (x < tau) {
-                tmpw += fexp(x * x / -0.18f);
-                count += 1.0f;
-            }
-            cf += (x > 1.2f);
-        }
-        float p = 0.f;
-        if (count > 2.0f && cf < 3)
-            p = tmpw / count;
-        else if (cf < 3)
-            p = fexp(tau * tau / -0.32f);
-        p = p * pri;
-        s_cdf[v][threadIdx.x] = p;
-        psum += p;
-    }
-    const float inv = 1.0f / psum;
-    float cum = 0.f;
-    for (int v = 0; v < nv; ++v) {
-        cum += s_cdf[v][threadIdx.x] * inv;
-        s_cdf[v][threadIdx.x] = cum;
-    }
-    float rp[15];
-    float u_depth_rand;
-    {
-        uint32_t o[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            rng.block(b, o);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int s = 4 * b + j;
-                if (s < 15)
-                    rp[s] = u01(o[j]) - FLT_EPSILON;
-                else
-                    u_depth_rand = u01(o[j]);
-            }
-        }
-    }
-    uint32_t done = 0, selmask = 0;
-    uint32_t wnib[4] = {0u, 0u, 0u, 0u};  // 4-bit draw counts per view
-    float wsum = 0.f;
-    for (int v = 0; v < nv; ++v) {
-        const float cd = s_cdf[v][threadIdx.x];
-        uint32_t hit = 0;
-#pragma unroll
-        for (int d = 0; d < 15; ++d) hit |= (uint32_t)(cd > rp[d]) << d;
-        const uint32_t fresh = hit & ~done;
-        done |= fresh;
-        const uint32_t cnt = __popc(fresh);
-        if (cnt) {
-            selmask |= 1u << v;
-            const uint32_t sh = 4u * (uint32_t)(v & 7);
-            const uint32_t add = cnt << sh;
-            if ((v >> 3) == 0) wnib[0] |= add;
-            else if ((v >> 3) == 1) wnib[1] |= add;
-            else if ((v >> 3) == 2) wnib[2] |= add;
-            else wnib[3] |= add;
-            wsum += (float)cnt;
-        }
-    }
-    auto weight_of = [&](int v) -> float {
-        const uint32_t wd = (v >> 3) == 0 ? wnib[0] : (v >> 3) == 1 ? wnib[1] : (v >> 3) == 2 ? wnib[2] : wnib[3];
-        return (float)((wd >> (4u * (uint32_t)(v & 7))) & 15u);
-    };
-
-    // ---- candidate final costs (src 983-1009)
-    float fc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const bool fk = (flags >> k) & 1u;
-        float4 cp = make_float4(0.f, 0.f, 0.f, 1.f);
-        if (fk && P.geom) cp = cplane[(size_t)k * NA + aidx];
-        float s = 0.f;
-        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
-            const int v = __ffs(mm) - 1;
-            const float wv = weight_of(v);
-            const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
-            if (P.geom && fk)
-                s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cp, fpx, fpy));
-            else
-                s += wv * x;
-        }
-        fc[k] = s / wsum;
-    }
-    const int kmin = argmin_last8(fc);
-
-    Hoist<NT, L> hs;
-    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub);
-
-    // Weighted cost of a hypothesis over the selected views (+ geometric term).
-    auto wcost = [&](float4 pl) -> float {
-        float m[3];
-        plane_m(rc, pl, m);
-        float s = 0.f;
-        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
-            const int v = __ffs(mm) - 1;
-            const ViewDesc& vd = s_views[v];
-            float H[9];
-            homography(vd.A, vd.b, m, H);
-            const float x = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
-            const float wv = weight_of(v);
-            if (P.geom)
-                s += wv * (x + 0.1f * geom_cost(rc, vd, pl, fpx, fpy));
-            else
-                s += wv * x;
-        }
-        return s / wsum;
-    };
-
-    // The current hypothesis re-costed with the new view weights; its NCCs
-    // come from pm_candidates (row 8 of the record), views with zero weight
-    // add exactly zero and are skipped.
-    float4 cur = planes[c];
-    float cnow;
-    {
-        float s = 0.f;
-        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
-            const int v = __ffs(mm) - 1;
-            const float x = ca[((size_t)8 * nv + v) * NA + aidx];
-            const float wv = weight_of(v);
-            if (P.geom)
-                s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cur, fpx, fpy));
-            else
-                s += wv * x;
-        }
-        cnow = s / wsum;
-    }
-    float cost = cnow;
-    uint32_t selc = sel[c];
-    float depth_now = depth_from_plane(rc, cur, fpx, fpy);
-    const float4 cur0 = cur;
-
-    const bool masked = P.prior && P.plane_masks[c] > 0u;
-    float4 pp = make_float4(0.f, 0.f, 0.f, 0.f);
-    float dprior = 0.f;
-    if (P.prior) {
-        pp = P.prior_planes[c];
-        dprior = depth_from_plane(rc, pp, fpx, fpy);
-    }
-    float restricted = 0.f;
-    if (P.prior && masked) {
-        float rs[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            rs[k] = 0.f;
-            if ((flags >> k) & 1u) {
-                const float4 cp = cplane[(size_t)k * NA + aidx];
-                rs[k] = prior_score(P, fc[k], depth_from_plane(rc, cp, fpx, fpy), dprior, pp, cp);
-            }
-        }
-        const int kmax = argmax_last8(rs);
-        const float rnow = prior_score(P, cnow, depth_from_plane(rc, cur0, fpx, fpy), dprior, pp, cur0);
-        if ((flags >> kmax) & 1u) {
-            const float4 cp = cplane[(size_t)kmax * NA + aidx];
-            const float db = depth_from_plane(rc, cp, fpx, fpy);
-            if (db >= P.dmin && db <= P.dmax && rs[kmax] > rnow) {
-                depth_now = db;
-                cur = cp;
-                cost = fc[kmax];
-                restricted = rs[kmax];
-                selc = selmask;
-            }
-        }
-    } else if ((flags >> kmin) & 1u) {
-        const float4 cp = cplane[(size_t)kmin * NA + aidx];
-        const float db = depth_from_plane(rc, cp, fpx, fpy);
-        if (db >= P.dmin && db <= P.dmax && fc[kmin] < cnow) {
-            depth_now = db;
-            cur = cp;
-            cost = fc[kmin];
-            if (!P.prior) selc = selmask;  // the prior path leaves it (lib 1306-1318)
-        }
-    }
-
-    // ---- refinement (src 595-676, lib 684-811)
-    const float pert = 0.02f;
-    float drand;
-    float4 nrand;
-    uint32_t o4[4], o5[4];
-    rng.block(4, o4);  // slots 16..19
-    if (P.prior && masked) {
-        const float dsig = (P.dmax - P.dmin) / 64.0f;
-        drand = u_depth_rand * 6.0f * dsig + (dprior - 3.0f * dsig);
-        rng.block(5, o5);  // slots 20..23
-        nrand = perturbed_normal(rc, fpx, fpy, pp, u01(o5[0]), u01(o5[1]), u01(o5[2]),
-                                 kPiF * (5.0f / 180.0f));
-    } else {
-        drand = u_depth_rand * (P.dmax - P.dmin) + P.dmin;
-        nrand = random_normal(rc, fpx, fpy, rng, RNG_NORMAL, depth_now);
-    }
-    const float dlo = (1.0f - pert) * depth_now;
-    const float dhi = (1.0f + pert) * depth_now;
-    const float dpert = u01(o4[0]) * (dhi - dlo) + dlo;
-    const float4 ncur = cur;
-    const float4 npert = perturbed_normal(rc, fpx, fpy, ncur, u01(o4[1]), u01(o4[2]), u01(o4[3]),
-                                          pert * kPiF);
-    const float dcur = depth_now;
-#pragma unroll 1
-    for (int i = 0; i < 5; ++i) {
-        const float di = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dcur);
-        float4 tp = (i == 0 || i == 4) ? ncur : (i == 3 ? npert : nrand);
-        tp.w = dist_to_origin(rc, fpx, fpy, di, tp);
-        const float tc = wcost(tp);
-        const float db = depth_from_plane(rc, tp, fpx, fpy);
-        if (P.prior && masked) {
-            const float sc = prior_score(P, tc, di, dprior, pp, tp);
-            if (db >= P.dmin && db <= P.dmax && sc > restricted) {
-                depth_now = db;
-                cur = tp;
-                cost = tc;
-                restricted = sc;
-            }
-        } else if (db >= P.dmin && db <= P.dmax && tc < cost) {
-            depth_now = db;
-            cur = tp;
-            cost = tc;
-        }
-    }
-    if (a.sub == 0) {
-        planes[c] = cur;
-        costs[c] = cost;
-        sel[c] = selc;
-        const uint32_t ns = (uint32_t)__popc(selmask);
-        ncc_count = 5u * ns;  // the 5 refinement candidates
-        geom_count = P.geom ? ((uint32_t)__popc(flags) + 6u) * ns : 0u;
-    }
+__device__ inline float nib_weight(const uint32_t wnib[4], int v) {
+    const uint32_t wd = (v >> 3) == 0 ? wnib[0] : (v >> 3) == 1 ? wnib[1] : (v >> 3) == 2 ? wnib[2] : wnib[3];
+    return (float)((wd >> (4u * (uint32_t)(v & 7))) & 15u);
 }
 
-template <int NT, int L>
-__global__ __launch_bounds__(kBlock) void pm_select_refine(
+// The active pixel with index aidx (row-major over the half-sweep's columns).
+__device__ inline Active active_of_index(const PMConst& P, int color, int aidx) {
+    Active a;
+    a.sub = 0;
+    a.aidx = aidx;
+    a.py = aidx / P.wh;
+    const int ax = aidx - a.py * P.wh;
+    a.px = 2 * ax + ((a.py + color) & 1);
+    a.ok = (a.py < P.h) && (a.px < P.w);
+    return a;
+}
+
+__global__ __launch_bounds__(kBlock) void pm_select(
     PMConst P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
     uint32_t* __restrict__ sel, const float* __restrict__ ca, const float4* __restrict__ cplane,
-    const uint8_t* __restrict__ cflag, int NA) {
-    __shared__ float s_cdf[kMaxViews][kBlock];
-    __shared__ ViewDesc s_views[kMaxViews];
-    {
+    const uint8_t* __restrict__ cflag, float4* __restrict__ rtp, uint32_t* __restrict__ rsw,
+    int NA) {
+    extern __shared__ __align__(16) unsigned char sel_dyn[];
+    ViewDesc* s_views = reinterpret_cast<ViewDesc*>(sel_dyn);
+    float* s_cdf = reinterpret_cast<float*>(sel_dyn + sizeof(ViewDesc) * (size_t)P.nv);  // [nv][kBlock]
+    if (P.geom) {
         const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
         const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
         uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
         for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
         __syncthreads();
     }
+    uint32_t geom_count = 0;
+    const int aidx = blockIdx.x * kBlock + threadIdx.x;
+    const Active a = active_of_index(P, color, aidx);
+    if (aidx < NA && a.ok) {
+        const int px = a.px, py = a.py;
+        const int c = py * P.w + px;
+        const float fpx = (float)px, fpy = (float)py;
+        const RefCam& rc = P.rc;
+        const int nv = P.nv;
+        const uint32_t flags = cflag[aidx];
+        const Rng rng{(uint32_t)c, (uint32_t)iter, P.seed, P.view};
+        float* cdf = s_cdf + threadIdx.x;
+
+        // ---- joint view selection (src 915-981)
+        uint32_t nbsel[4];
+        const int nb[4] = {c - P.w, c + P.w, c - 1, c + 1};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) nbsel[i] = ((flags >> (2 * i)) & 1u) ? sel[nb[i]] : 0u;
+        const float tau = 0.8f * fexp((float)(iter * iter) / -90.0f);
+        float psum = 0.f;
+        for (int v = 0; v < nv; ++v) {
+            float pri = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if ((flags >> (2 * i)) & 1u) pri += ((nbsel[i] >> v) & 1u) ? 0.9f : 0.1f;
+            float count = 0.f, tmpw = 0.f;
+            int cf = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
+                if (x < tau) {
+                    tmpw += fexp(x * x / -0.18f);
+                    count += 1.0f;
+                }
+                cf += (x > 1.2f);
+            }
+            float p = 0.f;
+            if (count > 2.0f && cf < 3)
+                p = tmpw / count;
+            else if (cf < 3)
+                p = fexp(tau * tau / -0.32f);
+            p = p * pri;
+            cdf[v * kBlock] = p;
+            psum += p;
+        }
+        const float inv = 1.0f / psum;
+        float cum = 0.f;
+        for (int v = 0; v < nv; ++v) {
+            cum += cdf[v * kBlock] * inv;
+            cdf[v * kBlock] = cum;
+        }
+        float rp[15];
+        float u_depth_rand;
+        {
+            uint32_t o[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                rng.block(b, o);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int s = 4 * b + j;
+                    if (s < 15)
+                        rp[s] = u01(o[j]) - FLT_EPSILON;
+                    else
+                        u_depth_rand = u01(o[j]);
+                }
+            }
+        }
+        uint32_t done = 0, selmask = 0;
+        uint32_t wnib[4] = {0u, 0u, 0u, 0u};
+        float wsum = 0.f;
+        for (int v = 0; v < nv; ++v) {
+            const float cd = cdf[v * kBlock];
+            uint32_t hit = 0;
+#pragma unroll
+            for (int d = 0; d < 15; ++d) hit |= (uint32_t)(cd > rp[d]) << d;
+            const uint32_t fresh = hit & ~done;
+            done |= fresh;
+            const uint32_t cnt = __popc(fresh);
+            if (cnt) {
+                selmask |= 1u << v;
+                const uint32_t add = cnt << (4u * (uint32_t)(v & 7));
+                if ((v >> 3) == 0) wnib[0] |= add;
+                else if ((v >> 3) == 1) wnib[1] |= add;
+                else if ((v >> 3) == 2) wnib[2] |= add;
+                else wnib[3] |= add;
+                wsum += (float)cnt;
+            }
+        }
+
+        // ---- candidate final costs (src 983-1009)
+        float fc[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool fk = (flags >> k) & 1u;
+            float4 cp = make_float4(0.f, 0.f, 0.f, 1.f);
+            if (fk && P.geom) cp = cplane[(size_t)k * NA + aidx];
+            float s = 0.f;
+            for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+                const int v = __ffs(mm) - 1;
+                const float wv = nib_weight(wnib, v);
+                const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
+                if (P.geom && fk)
+                    s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cp, fpx, fpy));
+                else
+                    s += wv * x;
+            }
+            fc[k] = s / wsum;
+        }
+        const int kmin = argmin_last8(fc);
+
+        // The current hypothesis re-costed with the new view weights; its
+        // NCCs come from pm_candidates (row 8 of the record).
+        float4 cur = planes[c];
+        float cnow;
+        {
+            float s = 0.f;
+            for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+                const int v = __ffs(mm) - 1;
+                const float x = ca[((size_t)8 * nv + v) * NA + aidx];
+                const float wv = nib_weight(wnib, v);
+                if (P.geom)
+                    s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cur, fpx, fpy));
+                else
+                    s += wv * x;
+            }
+            cnow = s / wsum;
+        }
+        float cost = cnow;
+        uint32_t selc = sel[c];
+        float depth_now = depth_from_plane(rc, cur, fpx, fpy);
+        const float4 cur0 = cur;
+
+        const bool masked = P.prior && P.plane_masks[c] > 0u;
+        float4 pp = make_float4(0.f, 0.f, 0.f, 0.f);
+        float dprior = 0.f;
+        if (P.prior) {
+            pp = P.prior_planes[c];
+            dprior = depth_from_plane(rc, pp, fpx, fpy);
+        }
+        float restricted = 0.f;
+        if (P.prior && masked) {
+            float rs[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                rs[k] = 0.f;
+                if ((flags >> k) & 1u) {
+                    const float4 cp = cplane[(size_t)k * NA + aidx];
+                    rs[k] = prior_score(P, fc[k], depth_from_plane(rc, cp, fpx, fpy), dprior, pp, cp);
+                }
+            }
+            const int kmax = argmax_last8(rs);
+            const float rnow = prior_score(P, cnow, depth_from_plane(rc, cur0, fpx, fpy), dprior, pp, cur0);
+            if ((flags >> kmax) & 1u) {
+                const float4 cp = cplane[(size_t)kmax * NA + aidx];
+                const float db = depth_from_plane(rc, cp, fpx, fpy);
+                if (db >= P.dmin && db <= P.dmax && rs[kmax] > rnow) {
+                    depth_now = db;
+                    cur = cp;
+                    cost = fc[kmax];
+                    restricted = rs[kmax];
+                    selc = selmask;
+                }
+            }
+        } else if ((flags >> kmin) & 1u) {
+            const float4 cp = cplane[(size_t)kmin * NA + aidx];
+            const float db = depth_from_plane(rc, cp, fpx, fpy);
+            if (db >= P.dmin && db <= P.dmax && fc[kmin] < cnow) {
+                depth_now = db;
+                cur = cp;
+                cost = fc[kmin];
+                if (!P.prior) selc = selmask;  // the prior path leaves it (lib 1306-1318)
+            }
+        }
+
+        // ---- refinement hypotheses (src 595-676, lib 684-811)
+        const float pert = 0.02f;
+        float drand;
+        float4 nrand;
+        uint32_t o4[4], o5[4];
+        rng.block(4, o4);  // slots 16..19
+        if (P.prior && masked) {
+            const float dsig = (P.dmax - P.dmin) / 64.0f;
+            drand = u_depth_rand * 6.0f * dsig + (dprior - 3.0f * dsig);
+            rng.block(5, o5);  // slots 20..23
+            nrand = perturbed_normal(rc, fpx, fpy, pp, u01(o5[0]), u01(o5[1]), u01(o5[2]),
+                                     kPiF * (5.0f / 180.0f));
+        } else {
+            drand = u_depth_rand * (P.dmax - P.dmin) + P.dmin;
+            nrand = random_normal(rc, fpx, fpy, rng, RNG_NORMAL, depth_now);
+        }
+        const float dlo = (1.0f - pert) * depth_now;
+        const float dhi = (1.0f + pert) * depth_now;
+        const float dpert = u01(o4[0]) * (dhi - dlo) + dlo;
+        const float4 npert = perturbed_normal(rc, fpx, fpy, cur, u01(o4[1]), u01(o4[2]), u01(o4[3]),
+                                              pert * kPiF);
+        const float dcur = depth_now;
+#pragma unroll
+        for (int i = 0; i < kRefine; ++i) {
+            const float di = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dcur);
+            float4 tp = (i == 0 || i == 4) ? cur : (i == 3 ? npert : nrand);
+            tp.w = dist_to_origin(rc, fpx, fpy, di, tp);
+            rtp[(size_t)i * NA + aidx] = tp;
+        }
+        planes[c] = cur;
+        costs[c] = cost;
+        sel[c] = selc;
+        rsw[(size_t)SW_SELMASK * NA + aidx] = selmask;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rsw[(size_t)(SW_WNIB0 + i) * NA + aidx] = wnib[i];
+        rsw[(size_t)SW_RESTRICTED * NA + aidx] = __float_as_uint(restricted);
+        rsw[(size_t)SW_DRAND * NA + aidx] = __float_as_uint(drand);
+        rsw[(size_t)SW_DPERT * NA + aidx] = __float_as_uint(dpert);
+        rsw[(size_t)SW_DCUR * NA + aidx] = __float_as_uint(dcur);
+        if (P.geom) geom_count = ((uint32_t)__popc(flags) + 1u) * (uint32_t)__popc(selmask);
+    }
+    if (P.work) count_work(P.work + 2 * K_SELECT, 0u, geom_count);
+}
+
+// Pixel `slot` of a pm_refine tile (slot = lane group in the NCC layout).
+__device__ inline Active active_slot(const PMConst& P, int tile, int color, int slot) {
+    using TL = Tiling<kLanes>;
+    const int tx = (P.wh + TL::TX - 1) / TL::TX;
+    const int ax = (tile % tx) * TL::TX + (slot % TL::TX);
+    const int y = (tile / tx) * TL::TY + (slot / TL::TX);
+    Active a;
+    a.sub = 0;
+    a.py = y;
+    a.px = 2 * ax + ((y + color) & 1);
+    a.aidx = y * P.wh + ax;
+    a.ok = (y < P.h) && (ax < P.wh) && (a.px < P.w);
+    return a;
+}
+
+template <int NT>
+struct RefineSmem {
+    static constexpr int NW = TapBlock<NT, kLanes>::NR * TapBlock<NT, kLanes>::NC;
+    float hw[kK2Pix][kLanes][2 * NW];  // hoisted w and w * (r - c) per (pixel, lane)
+    float hp[kK2Pix][4];               // 1 / sum w, mean_r, var_r, centre intensity
+    float4 tp[kRefine][kK2Pix];        // refinement hypotheses
+    int nunits;
+};
+
+// Dynamic LDS of pm_refine: ViewDesc[nv], uint16 units[64 * nv] (slot |
+// view << 6), float results[kRefine][64][nv].
+__host__ __device__ inline size_t refine_units_off(int nv) { return sizeof(ViewDesc) * (size_t)nv; }
+__host__ __device__ inline size_t refine_res_off(int nv) {
+    return (refine_units_off(nv) + sizeof(uint16_t) * (size_t)kK2Pix * nv + 15) & ~size_t(15);
+}
+__host__ __device__ inline size_t refine_dyn_bytes(int nv) {
+    return refine_res_off(nv) + sizeof(float) * (size_t)kRefine * kK2Pix * nv;
+}
+
+template <int NT, int L>
+__global__ __launch_bounds__(kBlock) void pm_refine(
+    PMConst P, int color, float4* __restrict__ planes, float* __restrict__ costs,
+    const float4* __restrict__ rtp, const uint32_t* __restrict__ rsw, int NA) {
+    static_assert(L == kLanes, "pm_refine is laid out for kLanes-wide groups");
+    using B = TapBlock<NT, L>;
+    constexpr int NW = RefineSmem<NT>::NW;
+    __shared__ RefineSmem<NT> S;
+    extern __shared__ __align__(16) unsigned char ref_dyn[];
+    ViewDesc* s_views = reinterpret_cast<ViewDesc*>(ref_dyn);
+    uint16_t* units = reinterpret_cast<uint16_t*>(ref_dyn + refine_units_off(P.nv));
+    float* res = reinterpret_cast<float*>(ref_dyn + refine_res_off(P.nv));
+    int tile;
+    if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
+    const RefCam& rc = P.rc;
+    const int g = threadIdx.x / L, sub = threadIdx.x % L;
+    {
+        const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
+        for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+    }
+    // the tile's refinement hypotheses
+    for (int i = threadIdx.x; i < kK2Pix * kRefine; i += kBlock) {
+        const int sl = i % kK2Pix, f = i / kK2Pix;
+        const Active a = active_slot(P, tile, color, sl);
+        S.tp[f][sl] = a.ok ? rtp[(size_t)f * NA + a.aidx] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // hoisted reference side of this lane group's pixel
+    const Active ag = active_slot(P, tile, color, g);
+    if (ag.ok) {
+        Hoist<NT, L> hs;
+        hoist_ref<NT, L>(hs, P.ref, ag.px, ag.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            S.hw[g][sub][i] = hs.wt[i];
+            S.hw[g][sub][NW + i] = hs.wr[i];
+        }
+        if (sub == 0) {
+            S.hp[g][0] = hs.inv_w;
+            S.hp[g][1] = hs.mean_r;
+            S.hp[g][2] = hs.var_r;
+            S.hp[g][3] = hs.c;
+        }
+    }
+    __syncthreads();
+    // work units (slot, view), slot-major: wave 0 scans the view counts
+    if (threadIdx.x < kK2Pix) {
+        const int sl = threadIdx.x;
+        const Active a = active_slot(P, tile, color, sl);
+        const uint32_t mask = a.ok ? rsw[(size_t)SW_SELMASK * NA + a.aidx] : 0u;
+        const int n = __popc(mask);
+        int incl = n;
+#pragma unroll
+        for (int off = 1; off < kK2Pix; off <<= 1) {
+            const int t = __shfl_up(incl, off);
+            if (sl >= off) incl += t;
+        }
+        int o = incl - n;
+        for (uint32_t mm = mask; mm; mm &= mm - 1u)
+            units[o++] = (uint16_t)(sl | ((__ffs(mm) - 1) << 6));
+        if (sl == kK2Pix - 1) S.nunits = incl;
+    }
+    __syncthreads();
+
     uint32_t ncc = 0, geom = 0;
-    pm_select_refine_body<NT, L>(P, color, iter, planes, costs, sel, ca, cplane, cflag, NA, s_cdf,
-                                 s_views, ncc, geom);
-    if (P.work) count_work(P.work + 2 * K_SELECT_REFINE, ncc, geom);
+    const int nunits = S.nunits;
+    for (int u = g; u < nunits; u += kK2Pix) {
+        const uint32_t code = units[u];
+        const int sl = (int)(code & 63u), v = (int)(code >> 6);
+        const Active a = active_slot(P, tile, color, sl);
+        const float fpx = (float)a.px, fpy = (float)a.py;
+        Hoist<NT, L> hs;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            hs.wt[i] = S.hw[sl][sub][i];
+            hs.wr[i] = S.hw[sl][sub][NW + i];
+        }
+#pragma unroll
+        for (int r = 0; r < B::NR; ++r)
+            hs.tx[r] = (float)min(max(a.px - P.radius + B::row(sub, r) * P.inc, 0), P.w - 1);
+#pragma unroll
+        for (int q = 0; q < B::NC; ++q)
+            hs.ty[q] = (float)min(max(a.py - P.radius + B::col(sub, q) * P.inc, 0), P.h - 1);
+        hs.inv_w = S.hp[sl][0];
+        hs.mean_r = S.hp[sl][1];
+        hs.var_r = S.hp[sl][2];
+        hs.c = S.hp[sl][3];
+        hs.px = fpx;
+        hs.py = fpy;
+        const ViewDesc& vd = s_views[v];
+#pragma unroll 1
+        for (int k = 0; k < kRefine; ++k) {
+            const float4 tp = S.tp[k][sl];
+            float m[3];
+            plane_m(rc, tp, m);
+            float H[9];
+            homography(vd.A, vd.b, m, H);
+            float x = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+            if (P.geom) x = x + 0.1f * geom_cost(rc, vd, tp, fpx, fpy);
+            if (sub == 0) res[(k * kK2Pix + sl) * P.nv + v] = x;
+        }
+        if (sub == 0) {
+            ncc += kRefine;
+            if (P.geom) geom += kRefine;
+        }
+    }
+    __syncthreads();
+
+    // sequential adoption over the 5 hypotheses, weighted in view order
+    if (threadIdx.x < kK2Pix) {
+        const int sl = threadIdx.x;
+        const Active a = active_slot(P, tile, color, sl);
+        if (a.ok) {
+            const int c = a.py * P.w + a.px;
+            const float fpx = (float)a.px, fpy = (float)a.py;
+            const uint32_t* rec = rsw + a.aidx;
+            const uint32_t mask = rec[(size_t)SW_SELMASK * NA];
+            uint32_t wnib[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) wnib[i] = rec[(size_t)(SW_WNIB0 + i) * NA];
+            float wsum = 0.f;
+            for (uint32_t mm = mask; mm; mm &= mm - 1u) wsum += nib_weight(wnib, __ffs(mm) - 1);
+            const bool pmask = P.prior && P.plane_masks[c] > 0u;
+            float4 pp = make_float4(0.f, 0.f, 0.f, 0.f);
+            float dprior = 0.f;
+            if (P.prior) {
+                pp = P.prior_planes[c];
+                dprior = depth_from_plane(rc, pp, fpx, fpy);
+            }
+            const float drand = __uint_as_float(rec[(size_t)SW_DRAND * NA]);
+            const float dpert = __uint_as_float(rec[(size_t)SW_DPERT * NA]);
+            const float dcur = __uint_as_float(rec[(size_t)SW_DCUR * NA]);
+            float restricted = __uint_as_float(rec[(size_t)SW_RESTRICTED * NA]);
+            float cost = costs[c];
+            int best = -1;
+#pragma unroll 1
+            for (int i = 0; i < kRefine; ++i) {
+                const float4 tp = S.tp[i][sl];
+                float s = 0.f;
+                for (uint32_t mm = mask; mm; mm &= mm - 1u) {
+                    const int v = __ffs(mm) - 1;
+                    s += nib_weight(wnib, v) * res[(i * kK2Pix + sl) * P.nv + v];
+                }
+                const float tc = s / wsum;
+                const float di = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dcur);
+                const float db = depth_from_plane(rc, tp, fpx, fpy);
+                if (pmask) {
+                    const float sc = prior_score(P, tc, di, dprior, pp, tp);
+                    if (db >= P.dmin && db <= P.dmax && sc > restricted) {
+                        best = i;
+                        cost = tc;
+                        restricted = sc;
+                    }
+                } else if (db >= P.dmin && db <= P.dmax && tc < cost) {
+                    best = i;
+                    cost = tc;
+                }
+            }
+            if (best >= 0) {
+                planes[c] = S.tp[best][sl];
+                costs[c] = cost;
+            }
+        }
+    }
+    if (P.work) count_work(P.work + 2 * K_REFINE, ncc, geom);
 }
 
 // ------------------------------------------------------ finalisation ---
@@ -1004,7 +1207,7 @@ namespace {
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct PMLayout {
-    size_t views, quads, sel, ca, cplane, cflag, total;
+    size_t views, quads, sel, ca, cplane, cflag, rtp, rsw, total;
 };
 
 PMLayout pm_layout(int h, int w, int num_images, bool sweep) {
@@ -1026,8 +1229,12 @@ PMLayout pm_layout(int h, int w, int num_images, bool sweep) {
         o = align_up(o + sizeof(float4) * 8 * na);
         L.cflag = o;
         o = align_up(o + na);
+        L.rtp = o;
+        o = align_up(o + sizeof(float4) * kRefine * na);
+        L.rsw = o;
+        o = align_up(o + sizeof(uint32_t) * SW_NUM * na);
     } else {
-        L.ca = L.cplane = L.cflag = o;
+        L.ca = L.cplane = L.cflag = L.rtp = L.rsw = o;
     }
     L.total = o;
     return L;
@@ -1163,6 +1370,19 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
         return GMVS_ERR_NOSPACE;
     }
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    {
+        // pm_refine keeps per-view descriptors, work units and refinement
+        // results in LDS (~1.4 KB per source view on top of the static part)
+        int dev = 0, lds = 0;
+        GMVS_HIP_CHECK(hipGetDevice(&dev));
+        GMVS_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t need = sizeof(RefineSmem<kNT>) + refine_dyn_bytes(A.num_images - 1);
+        if (need > (size_t)lds) {
+            set_error("%d source views need %zu bytes of LDS per block; the device allows %d",
+                      A.num_images - 1, need, lds);
+            return GMVS_ERR_UNSUPPORTED;
+        }
+    }
     char* ws = reinterpret_cast<char*>(workspace);
     if (int rc = setup_views(A.num_images, A.cameras, A.images, A.depths, ws, L, st)) return rc;
 
@@ -1184,6 +1404,8 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
     float* ca = reinterpret_cast<float*>(ws + L.ca);
     float4* cplane = reinterpret_cast<float4*>(ws + L.cplane);
     uint8_t* cflag = reinterpret_cast<uint8_t*>(ws + L.cflag);
+    float4* rtp = reinterpret_cast<float4*>(ws + L.rtp);
+    uint32_t* rsw = reinterpret_cast<uint32_t*>(ws + L.rsw);
     const int na = h * P.wh;
 
     const int tiles_full = Tiling<kLanes>::tiles(w, h);
@@ -1191,14 +1413,20 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
                        A.costs, sel);
     GMVS_LAUNCH_CHECK();
     const int tiles_half = Tiling<kLanes>::tiles(P.wh, h);
+    const size_t sel_lds = sizeof(ViewDesc) * (size_t)P.nv + sizeof(float) * (size_t)P.nv * kBlock;
+    const size_t ref_lds = refine_dyn_bytes(P.nv);
+    const dim3 grid_sel((na + kBlock - 1) / kBlock);
     const dim3 grid_half(grid_for_tiles(tiles_half));
     for (int it = 0; it < A.params.max_iterations; ++it) {
         for (int color = 0; color < 2; ++color) {
             GMVS_LAUNCH(K_CANDIDATES, st, (pm_candidates<kNT, kLanes>), grid_half, dim3(kBlock), 0, st, P, color, planes,
                                A.costs, ca, cplane, cflag, na);
             GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_SELECT_REFINE, st, (pm_select_refine<kNT, kLanes>), grid_half, dim3(kBlock), 0, st, P, color, it,
-                               planes, A.costs, sel, ca, cplane, cflag, na);
+            GMVS_LAUNCH(K_SELECT, st, pm_select, grid_sel, dim3(kBlock), sel_lds, st, P, color, it,
+                        planes, A.costs, sel, ca, cplane, cflag, rtp, rsw, na);
+            GMVS_LAUNCH_CHECK();
+            GMVS_LAUNCH(K_REFINE, st, (pm_refine<kNT, kLanes>), grid_half, dim3(kBlock), ref_lds, st, P,
+                        color, planes, A.costs, rtp, rsw, na);
             GMVS_LAUNCH_CHECK();
         }
     }

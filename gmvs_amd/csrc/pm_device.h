@@ -375,6 +375,30 @@ typedef const f32x4 __attribute__((address_space(1))) gfloat4;
 
 __device__ inline gfloat4* as_global(const float4* p) { return (gfloat4*)p; }
 
+// The packed cells are read with structured buffer loads: a descriptor with a
+// 16-byte stride over [count] cells and a per-load record index, so a tap's
+// address is one FMA and one conversion (y * w + x is exact in FP32 for
+// images of up to 2^24 pixels, which the host checks).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ f32x4 struct_buffer_load_f32x4(rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                          int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v4f32");
+
+struct QuadSrc {
+    rsrc_t rsrc;  // cells [count], 16-byte records
+    int base;     // record index of the view's first cell
+};
+
+// gfx9 buffer descriptor word 3: 32-bit data format, no swizzle.
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ inline QuadSrc quad_src(const float4* cells, uint32_t count, int base) {
+    QuadSrc q;
+    q.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(cells), (short)16, (int)count,
+                                               kBufWord3);
+    q.base = base;
+    return q;
+}
+
 // One bilateral NCC of the hoisted pixel against one source view. The
 // source image is stored as packed bilinear cells (v00, v01-v00, v10-v00,
 // v11-v10-v01+v00) with the reference's edge clamping already applied, so a
@@ -382,16 +406,14 @@ __device__ inline gfloat4* as_global(const float4* p) { return (gfloat4*)p; }
 // a whole patch row before consuming them. All lanes of a group return the
 // same cost.
 template <int NT, int L>
-__device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9],
-                                 const float4* __restrict__ quad_ptr, int h, int w) {
+__device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+                                 int h, int w) {
     using B = TapBlock<NT, L>;
     const float rcz = __builtin_amdgcn_rcpf(fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8])));
     const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
     const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
     if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
-    gfloat4* quad = as_global(quad_ptr);
-    const float wm1 = (float)(w - 1), hm1 = (float)(h - 1);
-    const uint32_t uw = (uint32_t)w;
+    const float wm1 = (float)(w - 1), hm1 = (float)(h - 1), wf = (float)w;
     float S1 = 0.f, S2 = 0.f, X = 0.f;
 #pragma unroll
     for (int r = 0; r < B::NR; ++r) {
@@ -413,7 +435,7 @@ __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9],
             const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
             fx[j] = x - xf;
             fy[j] = y - yf;
-            q[j] = quad[__umul24((uint32_t)yf, uw) + (uint32_t)xf];
+            q[j] = struct_buffer_load_f32x4(src.rsrc, src.base + (int)(uint32_t)fmaf(yf, wf, xf), 0, 0, 0);
         }
 #pragma unroll
         for (int j = 0; j < B::NC; ++j) {
@@ -425,6 +447,61 @@ __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9],
             S2 = fmaf(ws, s, S2);
             X = fmaf(hs.wr[r * B::NC + j], s, X);
         }
+    }
+    S1 = group_sum<L>(S1);
+    S2 = group_sum<L>(S2);
+    X = group_sum<L>(X);
+    const float ms = S1 * hs.inv_w;
+    const float var_s = S2 * hs.inv_w - ms * ms;
+    if (hs.var_r < 1e-5f || var_s < 1e-5f) return 2.0f;
+    const float cov = X * hs.inv_w - hs.mean_r * ms;
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov * __builtin_amdgcn_rsqf(hs.var_r * var_s)));
+}
+
+// ncc_eval with every gather of the lane issued before the first one is
+// consumed: more loads in flight per wave, for hypotheses whose footprints
+// are scattered (random depths) and mostly miss the L1.
+template <int NT, int L>
+__device__ inline float ncc_eval_mlp(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+                                     int h, int w) {
+    using B = TapBlock<NT, L>;
+    const float rcz = __builtin_amdgcn_rcpf(fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8])));
+    const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
+    const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
+    if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
+    const float wm1 = (float)(w - 1), hm1 = (float)(h - 1), wf = (float)w;
+    f32x4 q[B::NR * B::NC];
+    float fx[B::NR * B::NC], fy[B::NR * B::NC];
+#pragma unroll
+    for (int r = 0; r < B::NR; ++r) {
+        const float au = fmaf(H[0], hs.tx[r], H[2]);
+        const float av = fmaf(H[3], hs.tx[r], H[5]);
+        const float az = fmaf(H[6], hs.tx[r], H[8]);
+#pragma unroll
+        for (int j = 0; j < B::NC; ++j) {
+            const float u = fmaf(H[1], hs.ty[j], au);
+            const float v = fmaf(H[4], hs.ty[j], av);
+            const float z = fmaf(H[7], hs.ty[j], az);
+            const float rz = __builtin_amdgcn_rcpf(z);
+            const float x = u * rz, y = v * rz;
+            const float xf = __builtin_amdgcn_fmed3f(floorf(x), 0.f, wm1);
+            const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
+            fx[r * B::NC + j] = x - xf;
+            fy[r * B::NC + j] = y - yf;
+            q[r * B::NC + j] =
+                struct_buffer_load_f32x4(src.rsrc, src.base + (int)(uint32_t)fmaf(yf, wf, xf), 0, 0, 0);
+        }
+    }
+    float S1 = 0.f, S2 = 0.f, X = 0.f;
+#pragma unroll
+    for (int t = 0; t < B::NR * B::NC; ++t) {
+        const float top = fmaf(fx[t], q[t].y, q[t].x);
+        const float dyv = fmaf(fx[t], q[t].w, q[t].z);
+        const float s = fmaf(fy[t], dyv, top) - hs.c;
+        const float ws = hs.wt[t] * s;
+        S1 += ws;
+        S2 = fmaf(ws, s, S2);
+        X = fmaf(hs.wr[t], s, X);
     }
     S1 = group_sum<L>(S1);
     S2 = group_sum<L>(S2);

@@ -55,6 +55,7 @@ struct PMConst {
     RefCam rc;
     const float* ref;             // reference intensities [H*W]
     const ViewDesc* views;        // [nv]
+    const float4* quads;          // packed cells of all views, [nv][H*W]
     const float* init_depth;      // INIT_DEPTH_NORMAL
     const float* init_normal;     // INIT_DEPTH_NORMAL, world frame
     const float4* prior_planes;   // planar prior
@@ -230,7 +231,7 @@ __device__ inline void pm_init_body(const PMConst& P, float4* __restrict__ plane
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+        const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
         s_c[v][threadIdx.x] = cv;
         valid += (cv < 2.0f);
     }
@@ -503,7 +504,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int color,
             const ViewDesc& vd = P.views[v];
             float H[9];
             homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+            const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
             if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
     }
@@ -517,7 +518,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int color,
             const ViewDesc& vd = P.views[v];
             float H[9];
             homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+            const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
             if (lead) ca[((size_t)8 * P.nv + v) * NA + a.aidx] = cv;
         }
     }
@@ -992,6 +993,7 @@ __global__ __launch_bounds__(kBlock) void pm_refine(
         hs.px = fpx;
         hs.py = fpy;
         const ViewDesc& vd = s_views[v];
+        const QuadSrc all_views = quad_src(P.quads, (uint32_t)(P.nv * P.h * P.w), v * P.h * P.w);
 #pragma unroll 1
         for (int k = 0; k < kRefine; ++k) {
             const float4 tp = S.tp[k][sl];
@@ -999,7 +1001,7 @@ __global__ __launch_bounds__(kBlock) void pm_refine(
             plane_m(rc, tp, m);
             float H[9];
             homography(vd.A, vd.b, m, H);
-            float x = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+            float x = ncc_eval_mlp<NT, L>(hs, H, all_views, P.h, P.w);
             if (P.geom) x = x + 0.1f * geom_cost(rc, vd, tp, fpx, fpy);
             if (sub == 0) res[(k * kK2Pix + sl) * P.nv + v] = x;
         }
@@ -1196,7 +1198,7 @@ __global__ __launch_bounds__(kBlock) void pm_cost_batch(PMConst P, int n,
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L>(hs, H, vd.quad, P.h, P.w);
+        const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
         if (sub == 0) out[(size_t)q * P.nv + v] = cv;
     }
 }
@@ -1254,7 +1256,9 @@ int check_cameras(int num_images, const gmvs_camera* cams) {
                    "num_images must be in [2, %d] (got %d)", kMaxViews + 1, num_images);
     const int h = cams[0].height, w = cams[0].width;
     GMVS_ARG_CHECK(h > 0 && w > 0, "reference image size must be positive (got %dx%d)", w, h);
-    GMVS_ARG_CHECK((long long)h * w < (1LL << 31) / 16, "image too large (%dx%d)", w, h);
+    // tap addresses are formed as y * w + x in FP32, exact up to 2^24 pixels
+    GMVS_ARG_CHECK((long long)h * w <= (1LL << 24), "image too large (%dx%d; at most 2^24 pixels)",
+                   w, h);
     for (int i = 1; i < num_images; ++i)
         GMVS_ARG_CHECK(cams[i].height == h && cams[i].width == w,
                        "all images of a problem must share the reference size (view %d is %dx%d, "
@@ -1388,6 +1392,7 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
 
     PMConst P = make_const(A.num_images, A.cameras, A.images[0],
                            reinterpret_cast<const ViewDesc*>(ws + L.views), A.params, radius);
+    P.quads = reinterpret_cast<const float4*>(ws + L.quads);
     P.init_depth = A.depths ? A.depths[0] : nullptr;
     P.init_normal = A.ref_normal;
     P.prior_planes = reinterpret_cast<const float4*>(A.prior_planes);
@@ -1479,6 +1484,7 @@ extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
     if (int rc = setup_views(num_images, cameras, images, nullptr, ws, L, st)) return rc;
     PMConst P = make_const(num_images, cameras, images[0],
                            reinterpret_cast<const ViewDesc*>(ws + L.views), *params, radius);
+    P.quads = reinterpret_cast<const float4*>(ws + L.quads);
     GMVS_LAUNCH(K_COST_BATCH, st, (pm_cost_batch<kNT, kLanes>), dim3((n * kLanes + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, n, xy,
                        reinterpret_cast<const float4*>(planes), costs);
     GMVS_LAUNCH_CHECK();

@@ -173,6 +173,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
+    ap.add_argument("--variants", default="fp16,bf16",
+                    help="16-bit sample precisions also timed on the same workload "
+                         "(comma list of fp16, bf16; empty for none)")
     ap.add_argument("--summarize-profile", metavar="DIR", default=None,
                     help="print a summary of rocprofv3 CSV output in DIR and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -211,10 +214,10 @@ def main() -> int:
     n, h, w = pipe.n, pipe.h, pipe.w
     ph_step = n * (2 * pixel_hypotheses(h, w, 8) + pixel_hypotheses(h, w, 2))
 
-    def step():
-        s1 = pipe.first_stage(iterations=8, planar_prior=True, superpixel_filter=True)
-        s2 = pipe.geom_pass(s1, iterations=2)
-        fd, _, _ = pipe.fuse(s2, geom_consistent=3)
+    def step(p=pipe):
+        s1 = p.first_stage(iterations=8, planar_prior=True, superpixel_filter=True)
+        s2 = p.geom_pass(s1, iterations=2)
+        fd, _, _ = p.fuse(s2, geom_consistent=3)
         return fd
 
     for i in range(args.warmup):
@@ -263,6 +266,26 @@ def main() -> int:
     log(f"NCC evaluations per step {total_ncc // args.steps}, "
         f"{total_ncc / (elapsed) / 1e9:.2f} G NCC/s wall, kernel time {total_kernel_ms / args.steps:.1f} ms/step")
 
+    # the same workload with 16-bit source samples (FP32 NCC sums)
+    variants = {}
+    for name in [v for v in args.variants.split(",") if v]:
+        vp = MultiViewPipeline(sc.images, cams, pairs, labels=labels, prior_threads=8,
+                               precision=N.PRECISIONS[name])
+        for i in range(args.warmup):
+            step(vp)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            vfd = step(vp)
+        torch.cuda.synchronize()
+        vt = time.perf_counter() - t0
+        vval = ph_step * args.steps / vt / 1e6
+        vfused = int((vfd > 0).sum().item())
+        variants[name] = {"value": round(vval, 2), "ms_per_step": round(vt / args.steps * 1e3, 2),
+                          "speedup_vs_fp32": round(vval / value, 3), "fused_points": vfused}
+        log(f"{name}: {variants[name]}")
+        del vp
+
     fusion = None
     if not args.no_fusion:
         log("fusion workload: 49 views at 1600x1200, 20 sources each")
@@ -308,6 +331,7 @@ def main() -> int:
                    "superpixels": 2000, "planes": 24, "pixel_hypotheses_per_step": ph_step,
                    "fused_points": fused},
         "fusion": fusion,
+        "precision_variants": variants,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

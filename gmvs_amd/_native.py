@@ -24,6 +24,9 @@ INIT_DEPTH_NORMAL = 1
 INIT_PRIOR = 2
 
 PREC_FP32 = 0
+PREC_FP16 = 1
+PREC_BF16 = 2
+PRECISIONS = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16}
 
 
 class CCamera(ctypes.Structure):

@@ -84,7 +84,7 @@ struct ViewDesc {
     float A[9];
     float b[3];
     float K[9], R[9], t[3], C[3];  // source camera for the geometric term
-    const float4* quad;            // packed bilinear cells [H*W]
+    const void* cells;             // packed bilinear cells [H*W] (FP32 or 16-bit)
     const float* depth;            // source depth map [H*W] or nullptr
 };
 
@@ -367,45 +367,108 @@ __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref
     hs.var_r = R2 * hs.inv_w - hs.mean_r * hs.mean_r;
 }
 
-// Device buffers are global memory; loading through an address-space-1
-// pointer lets the compiler emit global (not flat) loads, which only count
-// against vmcnt and so can stay in flight across LDS traffic.
 typedef float __attribute__((ext_vector_type(4))) f32x4;
-typedef const f32x4 __attribute__((address_space(1))) gfloat4;
+typedef int __attribute__((ext_vector_type(2))) i32x2;
+typedef float __attribute__((ext_vector_type(2))) f32x2;
+typedef _Float16 __attribute__((ext_vector_type(2))) f16x2;
+typedef __bf16 __attribute__((ext_vector_type(2))) bf16x2;
 
-__device__ inline gfloat4* as_global(const float4* p) { return (gfloat4*)p; }
-
-// The packed cells are read with structured buffer loads: a descriptor with a
-// 16-byte stride over [count] cells and a per-load record index, so a tap's
-// address is one FMA and one conversion (y * w + x is exact in FP32 for
-// images of up to 2^24 pixels, which the host checks).
+// The packed cells are read with structured buffer loads: a descriptor whose
+// stride is the cell size, over [count] cells, and a per-load record index,
+// so a tap's address is one FMA and one conversion (y * w + x is exact in
+// FP32 for images of up to 2^24 pixels, which the host checks).
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 __device__ f32x4 struct_buffer_load_f32x4(rsrc_t rsrc, int vindex, int voffset, int soffset,
                                           int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v4f32");
+__device__ i32x2 struct_buffer_load_i32x2(rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                          int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
 
 struct QuadSrc {
-    rsrc_t rsrc;  // cells [count], 16-byte records
+    rsrc_t rsrc;  // cells [count]
     int base;     // record index of the view's first cell
 };
 
 // gfx9 buffer descriptor word 3: 32-bit data format, no swizzle.
 constexpr int kBufWord3 = 0x00020000;
 
-__device__ inline QuadSrc quad_src(const float4* cells, uint32_t count, int base) {
+// Storage and arithmetic of the source samples (GMVS_PREC_*):
+//   PREC_F32  -- cell (v00, v01-v00, v10-v00, v11-v10-v01+v00) in FP32,
+//                16 bytes; bilinear by three FP32 FMAs (the plain path);
+//   PREC_F16 / PREC_BF16 -- the same cell in FP16 / BF16, 8 bytes, and the
+//                bilinear sample s - c as two 16-bit dot products with FP32
+//                accumulation: dot((v00, v01-v00), (1, fx)) +
+//                dot((v10-v00, d11), (fy, fx*fy)) - c. Integer intensities
+//                up to 2048 (FP16) / 256 (BF16) and their differences are
+//                stored exactly; the weights carry the 16-bit rounding.
+// The NCC moments are accumulated in FP32 about the centre intensity in
+// every mode.
+enum : int { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
+
+template <int PR>
+struct Cell;
+
+template <>
+struct Cell<PREC_F32> {
+    static constexpr int kBytes = 16;
+    f32x4 q;
+    __device__ void load(const QuadSrc& s, int idx) {
+        q = struct_buffer_load_f32x4(s.rsrc, s.base + idx, 0, 0, 0);
+    }
+    __device__ float sample(float fx, float fy, float c) const {
+        const float top = fmaf(fx, q.y, q.x);
+        const float dyv = fmaf(fx, q.w, q.z);
+        return fmaf(fy, dyv, top) - c;
+    }
+};
+
+template <>
+struct Cell<PREC_F16> {
+    static constexpr int kBytes = 8;
+    i32x2 q;
+    __device__ void load(const QuadSrc& s, int idx) {
+        q = struct_buffer_load_i32x2(s.rsrc, s.base + idx, 0, 0, 0);
+    }
+    __device__ float sample(float fx, float fy, float c) const {
+        // (elements copied out first: clang's __builtin_bit_cast of a vector
+        // element reads the vector's first element)
+        const int q0 = q.x, q1 = q.y;
+        const f16x2 a = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(1.0f, fx));
+        const f16x2 b = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(fy, fx * fy));
+        const float lo = __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, q1), b, -c, false);
+        return __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, q0), a, lo, false);
+    }
+};
+
+template <>
+struct Cell<PREC_BF16> {
+    static constexpr int kBytes = 8;
+    i32x2 q;
+    __device__ void load(const QuadSrc& s, int idx) {
+        q = struct_buffer_load_i32x2(s.rsrc, s.base + idx, 0, 0, 0);
+    }
+    __device__ float sample(float fx, float fy, float c) const {
+        const int q0 = q.x, q1 = q.y;  // see Cell<PREC_F16>::sample
+        const bf16x2 a = __builtin_convertvector((f32x2){1.0f, fx}, bf16x2);
+        const bf16x2 b = __builtin_convertvector((f32x2){fy, fx * fy}, bf16x2);
+        const float lo = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q1), b, -c, false);
+        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q0), a, lo, false);
+    }
+};
+
+template <int PR>
+__device__ inline QuadSrc quad_src(const void* cells, uint32_t count, int base) {
     QuadSrc q;
-    q.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(cells), (short)16, (int)count,
-                                               kBufWord3);
+    q.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(cells), (short)Cell<PR>::kBytes,
+                                               (int)count, kBufWord3);
     q.base = base;
     return q;
 }
 
-// One bilateral NCC of the hoisted pixel against one source view. The
-// source image is stored as packed bilinear cells (v00, v01-v00, v10-v00,
-// v11-v10-v01+v00) with the reference's edge clamping already applied, so a
-// tap is one 16-byte gather and three FMAs. Each lane issues the gathers of
-// a whole patch row before consuming them. All lanes of a group return the
-// same cost.
-template <int NT, int L>
+// One bilateral NCC of the hoisted pixel against one source view. A tap is
+// one gather of a packed cell and its bilinear sample. Each lane issues the
+// gathers of a whole patch row before consuming them. All lanes of a group
+// return the same cost.
+template <int NT, int L, int PR>
 __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
                                  int h, int w) {
     using B = TapBlock<NT, L>;
@@ -420,7 +483,7 @@ __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const
         const float au = fmaf(H[0], hs.tx[r], H[2]);
         const float av = fmaf(H[3], hs.tx[r], H[5]);
         const float az = fmaf(H[6], hs.tx[r], H[8]);
-        f32x4 q[B::NC];
+        Cell<PR> q[B::NC];
         float fx[B::NC], fy[B::NC];
 #pragma unroll
         for (int j = 0; j < B::NC; ++j) {
@@ -435,13 +498,11 @@ __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const
             const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
             fx[j] = x - xf;
             fy[j] = y - yf;
-            q[j] = struct_buffer_load_f32x4(src.rsrc, src.base + (int)(uint32_t)fmaf(yf, wf, xf), 0, 0, 0);
+            q[j].load(src, (int)(uint32_t)fmaf(yf, wf, xf));
         }
 #pragma unroll
         for (int j = 0; j < B::NC; ++j) {
-            const float top = fmaf(fx[j], q[j].y, q[j].x);
-            const float dyv = fmaf(fx[j], q[j].w, q[j].z);
-            const float s = fmaf(fy[j], dyv, top) - hs.c;
+            const float s = q[j].sample(fx[j], fy[j], hs.c);
             const float ws = hs.wt[r * B::NC + j] * s;
             S1 += ws;
             S2 = fmaf(ws, s, S2);
@@ -461,7 +522,7 @@ __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const
 // ncc_eval with every gather of the lane issued before the first one is
 // consumed: more loads in flight per wave, for hypotheses whose footprints
 // are scattered (random depths) and mostly miss the L1.
-template <int NT, int L>
+template <int NT, int L, int PR>
 __device__ inline float ncc_eval_mlp(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
                                      int h, int w) {
     using B = TapBlock<NT, L>;
@@ -470,7 +531,7 @@ __device__ inline float ncc_eval_mlp(const Hoist<NT, L>& hs, const float H[9], c
     const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
     if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
     const float wm1 = (float)(w - 1), hm1 = (float)(h - 1), wf = (float)w;
-    f32x4 q[B::NR * B::NC];
+    Cell<PR> q[B::NR * B::NC];
     float fx[B::NR * B::NC], fy[B::NR * B::NC];
 #pragma unroll
     for (int r = 0; r < B::NR; ++r) {
@@ -488,16 +549,13 @@ __device__ inline float ncc_eval_mlp(const Hoist<NT, L>& hs, const float H[9], c
             const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
             fx[r * B::NC + j] = x - xf;
             fy[r * B::NC + j] = y - yf;
-            q[r * B::NC + j] =
-                struct_buffer_load_f32x4(src.rsrc, src.base + (int)(uint32_t)fmaf(yf, wf, xf), 0, 0, 0);
+            q[r * B::NC + j].load(src, (int)(uint32_t)fmaf(yf, wf, xf));
         }
     }
     float S1 = 0.f, S2 = 0.f, X = 0.f;
 #pragma unroll
     for (int t = 0; t < B::NR * B::NC; ++t) {
-        const float top = fmaf(fx[t], q[t].y, q[t].x);
-        const float dyv = fmaf(fx[t], q[t].w, q[t].z);
-        const float s = fmaf(fy[t], dyv, top) - hs.c;
+        const float s = q[t].sample(fx[t], fy[t], hs.c);
         const float ws = hs.wt[t] * s;
         S1 += ws;
         S2 = fmaf(ws, s, S2);

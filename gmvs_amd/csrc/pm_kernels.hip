@@ -55,7 +55,7 @@ struct PMConst {
     RefCam rc;
     const float* ref;             // reference intensities [H*W]
     const ViewDesc* views;        // [nv]
-    const float4* quads;          // packed cells of all views, [nv][H*W]
+    const void* cells;            // packed cells of all views, [nv][H*W]
     const float* init_depth;      // INIT_DEPTH_NORMAL
     const float* init_normal;     // INIT_DEPTH_NORMAL, world frame
     const float4* prior_planes;   // planar prior
@@ -106,7 +106,7 @@ __host__ inline int grid_for_tiles(int ntiles) {
 // there, Kr^-1 uses fx, fy, cx, cy and Ks contributes K0, K2, K4, K5, K8).
 // One single-thread launch per source view; the cameras travel as kernel
 // arguments.
-__global__ void pm_setup_view(gmvs_camera r, gmvs_camera s, const float4* quad,
+__global__ void pm_setup_view(gmvs_camera r, gmvs_camera s, const void* cells,
                               const float* depth, ViewDesc* __restrict__ out) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double Cr[3], Cs[3];
@@ -144,15 +144,18 @@ __global__ void pm_setup_view(gmvs_camera r, gmvs_camera s, const float4* quad,
         d.t[i] = s.t[i];
         d.C[i] = -(s.R[i] * s.t[0] + s.R[3 + i] * s.t[1] + s.R[6 + i] * s.t[2]);
     }
-    d.quad = quad;
+    d.cells = cells;
     d.depth = depth;
     *out = d;
 }
 
 // Packs an f32 image into bilinear cells (v00, v01-v00, v10-v00,
 // v11-v10-v01+v00), with the right/bottom neighbours clamped to the last
-// column/row as the reference's software bilinear does (src 20-21).
-__global__ void pm_pack_quads(const float* __restrict__ img, float4* __restrict__ quad, int h,
+// column/row as the reference's software bilinear does (src 20-21). The
+// differences are formed in FP32, then stored in the cell precision
+// (round to nearest even).
+template <int PR>
+__global__ void pm_pack_quads(const float* __restrict__ img, void* __restrict__ cells, int h,
                               int w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= h * w) return;
@@ -161,11 +164,23 @@ __global__ void pm_pack_quads(const float* __restrict__ img, float4* __restrict_
     const int yu = (y == h - 1) ? y : y + 1;
     const float v0 = img[y * w + x], v1 = img[y * w + xu];
     const float v2 = img[yu * w + x], v3 = img[yu * w + xu];
-    quad[i] = make_float4(v0, v1 - v0, v2 - v0, (v3 - v2) - (v1 - v0));
+    const float d01 = v1 - v0, d10 = v2 - v0, d11 = (v3 - v2) - (v1 - v0);
+    if constexpr (PR == PREC_F32) {
+        reinterpret_cast<float4*>(cells)[i] = make_float4(v0, d01, d10, d11);
+    } else if constexpr (PR == PREC_F16) {
+        const f16x2 lo = {(_Float16)v0, (_Float16)d01}, hi = {(_Float16)d10, (_Float16)d11};
+        reinterpret_cast<uint2*>(cells)[i] =
+            make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+    } else {
+        const bf16x2 lo = __builtin_convertvector((f32x2){v0, d01}, bf16x2);
+        const bf16x2 hi = __builtin_convertvector((f32x2){d10, d11}, bf16x2);
+        reinterpret_cast<uint2*>(cells)[i] =
+            make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+    }
 }
 
 // ---------------------------------------------------------------- init ---
-template <int NT, int L>
+template <int NT, int L, int PR>
 __device__ inline void pm_init_body(const PMConst& P, float4* __restrict__ planes,
                                     float* __restrict__ costs, uint32_t* __restrict__ sel,
                                     float (*s_c)[kBlock], uint32_t& ncc) {
@@ -231,7 +246,7 @@ __device__ inline void pm_init_body(const PMConst& P, float4* __restrict__ plane
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
+        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
         s_c[v][threadIdx.x] = cv;
         valid += (cv < 2.0f);
     }
@@ -271,13 +286,13 @@ __device__ inline void pm_init_body(const PMConst& P, float4* __restrict__ plane
     }
 }
 
-template <int NT, int L>
+template <int NT, int L, int PR>
 __global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict__ planes,
                                                   float* __restrict__ costs,
                                                   uint32_t* __restrict__ sel) {
     __shared__ float s_c[kMaxViews][kBlock];
     uint32_t ncc = 0;
-    pm_init_body<NT, L>(P, planes, costs, sel, s_c, ncc);
+    pm_init_body<NT, L, PR>(P, planes, costs, sel, s_c, ncc);
     if (P.work) count_work(P.work + 2 * K_INIT, ncc, 0);
 }
 
@@ -474,7 +489,7 @@ __device__ inline void candidate_search(const float* __restrict__ cs, int px, in
     }
 }
 
-template <int NT, int L>
+template <int NT, int L, int PR>
 __device__ inline void pm_candidates_body(const PMConst& P, int color,
                                           const float4* __restrict__ planes,
                                           const float* __restrict__ costs, float* __restrict__ ca,
@@ -504,7 +519,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int color,
             const ViewDesc& vd = P.views[v];
             float H[9];
             homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
+            const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
             if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
     }
@@ -518,13 +533,13 @@ __device__ inline void pm_candidates_body(const PMConst& P, int color,
             const ViewDesc& vd = P.views[v];
             float H[9];
             homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
+            const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
             if (lead) ca[((size_t)8 * P.nv + v) * NA + a.aidx] = cv;
         }
     }
 }
 
-template <int NT, int L>
+template <int NT, int L, int PR>
 __global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
                                                         const float4* __restrict__ planes,
                                                         const float* __restrict__ costs,
@@ -532,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
                                                         float4* __restrict__ cplane,
                                                         uint8_t* __restrict__ cflag, int NA) {
     uint32_t ncc = 0;
-    pm_candidates_body<NT, L>(P, color, planes, costs, ca, cplane, cflag, NA, ncc);
+    pm_candidates_body<NT, L, PR>(P, color, planes, costs, ca, cplane, cflag, NA, ncc);
     if (P.work) count_work(P.work + 2 * K_CANDIDATES, ncc, 0);
 }
 
@@ -902,7 +917,7 @@ __host__ __device__ inline size_t refine_dyn_bytes(int nv) {
     return refine_res_off(nv) + sizeof(float) * (size_t)kRefine * kK2Pix * nv;
 }
 
-template <int NT, int L>
+template <int NT, int L, int PR>
 __global__ __launch_bounds__(kBlock) void pm_refine(
     PMConst P, int color, float4* __restrict__ planes, float* __restrict__ costs,
     const float4* __restrict__ rtp, const uint32_t* __restrict__ rsw, int NA) {
@@ -993,7 +1008,7 @@ __global__ __launch_bounds__(kBlock) void pm_refine(
         hs.px = fpx;
         hs.py = fpy;
         const ViewDesc& vd = s_views[v];
-        const QuadSrc all_views = quad_src(P.quads, (uint32_t)(P.nv * P.h * P.w), v * P.h * P.w);
+        const QuadSrc all_views = quad_src<PR>(P.cells, (uint32_t)(P.nv * P.h * P.w), v * P.h * P.w);
 #pragma unroll 1
         for (int k = 0; k < kRefine; ++k) {
             const float4 tp = S.tp[k][sl];
@@ -1001,7 +1016,7 @@ __global__ __launch_bounds__(kBlock) void pm_refine(
             plane_m(rc, tp, m);
             float H[9];
             homography(vd.A, vd.b, m, H);
-            float x = ncc_eval_mlp<NT, L>(hs, H, all_views, P.h, P.w);
+            float x = ncc_eval_mlp<NT, L, PR>(hs, H, all_views, P.h, P.w);
             if (P.geom) x = x + 0.1f * geom_cost(rc, vd, tp, fpx, fpy);
             if (sub == 0) res[(k * kK2Pix + sl) * P.nv + v] = x;
         }
@@ -1176,7 +1191,7 @@ __global__ void pm_export(int hw, const float4* __restrict__ planes, const float
 }
 
 // Parity hook: V NCCs for a list of (pixel, plane) pairs.
-template <int NT, int L>
+template <int NT, int L, int PR>
 __global__ __launch_bounds__(kBlock) void pm_cost_batch(PMConst P, int n,
                                                         const int32_t* __restrict__ xy,
                                                         const float4* __restrict__ planes,
@@ -1198,7 +1213,7 @@ __global__ __launch_bounds__(kBlock) void pm_cost_batch(PMConst P, int n,
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L>(hs, H, quad_src(vd.quad, P.h * P.w, 0), P.h, P.w);
+        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
         if (sub == 0) out[(size_t)q * P.nv + v] = cv;
     }
 }
@@ -1212,7 +1227,13 @@ struct PMLayout {
     size_t views, quads, sel, ca, cplane, cflag, rtp, rsw, total;
 };
 
-PMLayout pm_layout(int h, int w, int num_images, bool sweep) {
+bool valid_precision(int precision) {
+    return precision == GMVS_PREC_FP32 || precision == GMVS_PREC_FP16 || precision == GMVS_PREC_BF16;
+}
+
+size_t cell_bytes(int precision) { return precision == GMVS_PREC_FP32 ? 16 : 8; }
+
+PMLayout pm_layout(int h, int w, int num_images, bool sweep, int precision) {
     const int nv = num_images - 1;
     const size_t hw = size_t(h) * w;
     const size_t na = size_t(h) * ((w + 1) / 2);
@@ -1221,7 +1242,7 @@ PMLayout pm_layout(int h, int w, int num_images, bool sweep) {
     L.views = o;
     o = align_up(o + sizeof(ViewDesc) * nv);
     L.quads = o;
-    o = align_up(o + sizeof(float4) * hw * nv);
+    o = align_up(o + cell_bytes(precision) * hw * nv);
     L.sel = o;
     if (sweep) {
         o = align_up(o + sizeof(uint32_t) * hw);
@@ -1268,20 +1289,22 @@ int check_cameras(int num_images, const gmvs_camera* cams) {
 
 // Composes the per-view homography factors (FP64, one tiny launch per view)
 // and packs the source images.
+template <int PR>
 int setup_views(int num_images, const gmvs_camera* cams, const float* const* images,
                 const float* const* depths, char* ws, const PMLayout& L, hipStream_t st) {
     const int h = cams[0].height, w = cams[0].width;
     const int nv = num_images - 1;
     const size_t hw = size_t(h) * w;
-    float4* quads = reinterpret_cast<float4*>(ws + L.quads);
+    char* cells = ws + L.quads;
+    const size_t view_bytes = hw * Cell<PR>::kBytes;
     ViewDesc* views = reinterpret_cast<ViewDesc*>(ws + L.views);
     const int blocks = int((hw + 255) / 256);
     for (int v = 0; v < nv; ++v) {
         GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_view, dim3(1), dim3(64), 0, st, cams[0], cams[v + 1],
-                           quads + hw * v, depths ? depths[v + 1] : nullptr, views + v);
+                    (const void*)(cells + view_bytes * v), depths ? depths[v + 1] : nullptr, views + v);
         GMVS_LAUNCH_CHECK();
-        GMVS_LAUNCH(K_PACK, st, pm_pack_quads, dim3(blocks), dim3(256), 0, st, images[v + 1],
-                           quads + hw * v, h, w);
+        GMVS_LAUNCH(K_PACK, st, pm_pack_quads<PR>, dim3(blocks), dim3(256), 0, st, images[v + 1],
+                    (void*)(cells + view_bytes * v), h, w);
         GMVS_LAUNCH_CHECK();
     }
     return GMVS_OK;
@@ -1324,75 +1347,15 @@ PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, co
 
 using namespace gmvs;
 
-extern "C" size_t gmvs_pm_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
-                                          int32_t precision) {
-    if (height <= 0 || width <= 0 || num_images < 2 || precision != GMVS_PREC_FP32) return 0;
-    return pm_layout(height, width, num_images, true).total;
-}
-
-extern "C" size_t gmvs_cost_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
-                                            int32_t precision) {
-    if (height <= 0 || width <= 0 || num_images < 2 || precision != GMVS_PREC_FP32) return 0;
-    return pm_layout(height, width, num_images, false).total;
-}
-
-extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t workspace_bytes,
-                           void* stream) {
-    clear_error();
-    GMVS_ARG_CHECK(args, "args is NULL");
-    const gmvs_pm_args& A = *args;
-    if (int rc = check_cameras(A.num_images, A.cameras)) return rc;
-    GMVS_ARG_CHECK(A.images, "images is NULL");
-    for (int i = 0; i < A.num_images; ++i) GMVS_ARG_CHECK(A.images[i], "images[%d] is NULL", i);
-    GMVS_ARG_CHECK(A.planes && A.costs, "planes/costs state buffers are required");
-    GMVS_ARG_CHECK(A.precision == GMVS_PREC_FP32, "unsupported precision %d", A.precision);
-    GMVS_ARG_CHECK(A.init_mode >= 0 && A.init_mode <= 2, "bad init_mode %d", A.init_mode);
-    GMVS_ARG_CHECK(A.params.max_iterations >= 0, "max_iterations must be >= 0");
-    GMVS_ARG_CHECK(A.params.top_k >= 0, "top_k must be >= 0");
-    if (A.init_mode == GMVS_INIT_DEPTH_NORMAL)
-        GMVS_ARG_CHECK(A.depths && A.depths[0] && A.ref_normal,
-                       "INIT_DEPTH_NORMAL needs depths[0] and ref_normal");
-    if (A.geom) {
-        GMVS_ARG_CHECK(A.depths, "the geometric term needs source depth maps");
-        for (int i = 1; i < A.num_images; ++i) GMVS_ARG_CHECK(A.depths[i], "depths[%d] is NULL", i);
-    }
-    if (A.params.planar_prior)
-        GMVS_ARG_CHECK(A.prior_planes && A.plane_masks, "planar_prior needs prior planes and masks");
-    int radius = 0;
-    const int nt = taps_per_axis(A.params, &radius);
-    if (nt != 6) {
-        set_error("patch geometry (patch_size=%d, radius_increment=%d) gives %d taps per axis; "
-                  "this build is specialised for 6 (the reference defaults 11/2)",
-                  A.params.patch_size, A.params.radius_increment, nt);
-        return GMVS_ERR_UNSUPPORTED;
-    }
+// One pass (init, sweeps, depth/normal, filter, export) in cell precision PR.
+template <int PR>
+int run_pass(const gmvs_pm_args& A, char* ws, const PMLayout& L, int radius, hipStream_t st) {
     const int h = A.cameras[0].height, w = A.cameras[0].width;
-    const PMLayout L = pm_layout(h, w, A.num_images, true);
-    GMVS_ARG_CHECK(workspace, "workspace is NULL");
-    if (workspace_bytes < L.total) {
-        set_error("workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
-        return GMVS_ERR_NOSPACE;
-    }
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    {
-        // pm_refine keeps per-view descriptors, work units and refinement
-        // results in LDS (~1.4 KB per source view on top of the static part)
-        int dev = 0, lds = 0;
-        GMVS_HIP_CHECK(hipGetDevice(&dev));
-        GMVS_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = sizeof(RefineSmem<kNT>) + refine_dyn_bytes(A.num_images - 1);
-        if (need > (size_t)lds) {
-            set_error("%d source views need %zu bytes of LDS per block; the device allows %d",
-                      A.num_images - 1, need, lds);
-            return GMVS_ERR_UNSUPPORTED;
-        }
-    }
-    char* ws = reinterpret_cast<char*>(workspace);
-    if (int rc = setup_views(A.num_images, A.cameras, A.images, A.depths, ws, L, st)) return rc;
+    if (int rc = setup_views<PR>(A.num_images, A.cameras, A.images, A.depths, ws, L, st)) return rc;
 
     PMConst P = make_const(A.num_images, A.cameras, A.images[0],
                            reinterpret_cast<const ViewDesc*>(ws + L.views), A.params, radius);
-    P.quads = reinterpret_cast<const float4*>(ws + L.quads);
+    P.cells = ws + L.quads;
     P.init_depth = A.depths ? A.depths[0] : nullptr;
     P.init_normal = A.ref_normal;
     P.prior_planes = reinterpret_cast<const float4*>(A.prior_planes);
@@ -1414,7 +1377,7 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
     const int na = h * P.wh;
 
     const int tiles_full = Tiling<kLanes>::tiles(w, h);
-    GMVS_LAUNCH(K_INIT, st, (pm_init<kNT, kLanes>), dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
+    GMVS_LAUNCH(K_INIT, st, (pm_init<kNT, kLanes, PR>), dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
                        A.costs, sel);
     GMVS_LAUNCH_CHECK();
     const int tiles_half = Tiling<kLanes>::tiles(P.wh, h);
@@ -1424,13 +1387,13 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
     const dim3 grid_half(grid_for_tiles(tiles_half));
     for (int it = 0; it < A.params.max_iterations; ++it) {
         for (int color = 0; color < 2; ++color) {
-            GMVS_LAUNCH(K_CANDIDATES, st, (pm_candidates<kNT, kLanes>), grid_half, dim3(kBlock), 0, st, P, color, planes,
+            GMVS_LAUNCH(K_CANDIDATES, st, (pm_candidates<kNT, kLanes, PR>), grid_half, dim3(kBlock), 0, st, P, color, planes,
                                A.costs, ca, cplane, cflag, na);
             GMVS_LAUNCH_CHECK();
             GMVS_LAUNCH(K_SELECT, st, pm_select, grid_sel, dim3(kBlock), sel_lds, st, P, color, it,
                         planes, A.costs, sel, ca, cplane, cflag, rtp, rsw, na);
             GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_REFINE, st, (pm_refine<kNT, kLanes>), grid_half, dim3(kBlock), ref_lds, st, P,
+            GMVS_LAUNCH(K_REFINE, st, (pm_refine<kNT, kLanes, PR>), grid_half, dim3(kBlock), ref_lds, st, P,
                         color, planes, A.costs, rtp, rsw, na);
             GMVS_LAUNCH_CHECK();
         }
@@ -1453,6 +1416,95 @@ extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t wor
     return GMVS_OK;
 }
 
+template <int PR>
+int cost_batch(int num_images, const gmvs_camera* cameras, const float* const* images,
+               const gmvs_params& params, int radius, int n, const int32_t* xy, const float* planes,
+               float* costs, char* ws, const PMLayout& L, hipStream_t st) {
+    if (int rc = setup_views<PR>(num_images, cameras, images, nullptr, ws, L, st)) return rc;
+    PMConst P = make_const(num_images, cameras, images[0],
+                           reinterpret_cast<const ViewDesc*>(ws + L.views), params, radius);
+    P.cells = ws + L.quads;
+    GMVS_LAUNCH(K_COST_BATCH, st, (pm_cost_batch<kNT, kLanes, PR>),
+                dim3((n * kLanes + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, n, xy,
+                reinterpret_cast<const float4*>(planes), costs);
+    GMVS_LAUNCH_CHECK();
+    return GMVS_OK;
+}
+
+extern "C" size_t gmvs_pm_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
+                                          int32_t precision) {
+    if (height <= 0 || width <= 0 || num_images < 2 || !valid_precision(precision)) return 0;
+    return pm_layout(height, width, num_images, true, precision).total;
+}
+
+extern "C" size_t gmvs_cost_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
+                                            int32_t precision) {
+    if (height <= 0 || width <= 0 || num_images < 2 || !valid_precision(precision)) return 0;
+    return pm_layout(height, width, num_images, false, precision).total;
+}
+
+extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t workspace_bytes,
+                           void* stream) {
+    clear_error();
+    GMVS_ARG_CHECK(args, "args is NULL");
+    const gmvs_pm_args& A = *args;
+    if (int rc = check_cameras(A.num_images, A.cameras)) return rc;
+    GMVS_ARG_CHECK(A.images, "images is NULL");
+    for (int i = 0; i < A.num_images; ++i) GMVS_ARG_CHECK(A.images[i], "images[%d] is NULL", i);
+    GMVS_ARG_CHECK(A.planes && A.costs, "planes/costs state buffers are required");
+    GMVS_ARG_CHECK(valid_precision(A.precision), "unsupported precision %d", A.precision);
+    GMVS_ARG_CHECK(A.init_mode >= 0 && A.init_mode <= 2, "bad init_mode %d", A.init_mode);
+    GMVS_ARG_CHECK(A.params.max_iterations >= 0, "max_iterations must be >= 0");
+    GMVS_ARG_CHECK(A.params.top_k >= 0, "top_k must be >= 0");
+    if (A.init_mode == GMVS_INIT_DEPTH_NORMAL)
+        GMVS_ARG_CHECK(A.depths && A.depths[0] && A.ref_normal,
+                       "INIT_DEPTH_NORMAL needs depths[0] and ref_normal");
+    if (A.geom) {
+        GMVS_ARG_CHECK(A.depths, "the geometric term needs source depth maps");
+        for (int i = 1; i < A.num_images; ++i) GMVS_ARG_CHECK(A.depths[i], "depths[%d] is NULL", i);
+    }
+    if (A.params.planar_prior)
+        GMVS_ARG_CHECK(A.prior_planes && A.plane_masks, "planar_prior needs prior planes and masks");
+    int radius = 0;
+    const int nt = taps_per_axis(A.params, &radius);
+    if (nt != 6) {
+        set_error("patch geometry (patch_size=%d, radius_increment=%d) gives %d taps per axis; "
+                  "this build is specialised for 6 (the reference defaults 11/2)",
+                  A.params.patch_size, A.params.radius_increment, nt);
+        return GMVS_ERR_UNSUPPORTED;
+    }
+    const int h = A.cameras[0].height, w = A.cameras[0].width;
+    const PMLayout L = pm_layout(h, w, A.num_images, true, A.precision);
+    GMVS_ARG_CHECK(workspace, "workspace is NULL");
+    if (workspace_bytes < L.total) {
+        set_error("workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
+        return GMVS_ERR_NOSPACE;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    {
+        // pm_refine keeps per-view descriptors, work units and refinement
+        // results in LDS (~1.4 KB per source view on top of the static part)
+        int dev = 0, lds = 0;
+        GMVS_HIP_CHECK(hipGetDevice(&dev));
+        GMVS_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t need = sizeof(RefineSmem<kNT>) + refine_dyn_bytes(A.num_images - 1);
+        if (need > (size_t)lds) {
+            set_error("%d source views need %zu bytes of LDS per block; the device allows %d",
+                      A.num_images - 1, need, lds);
+            return GMVS_ERR_UNSUPPORTED;
+        }
+    }
+    char* ws = reinterpret_cast<char*>(workspace);
+    switch (A.precision) {
+        case GMVS_PREC_FP16:
+            return run_pass<PREC_F16>(A, ws, L, radius, st);
+        case GMVS_PREC_BF16:
+            return run_pass<PREC_BF16>(A, ws, L, radius, st);
+        default:
+            return run_pass<PREC_F32>(A, ws, L, radius, st);
+    }
+}
+
 extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
                                const float* const* images, const gmvs_params* params,
                                int32_t precision, int32_t n, const int32_t* xy,
@@ -1462,7 +1514,7 @@ extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
     if (int rc = check_cameras(num_images, cameras)) return rc;
     GMVS_ARG_CHECK(images && params, "images/params is NULL");
     for (int i = 0; i < num_images; ++i) GMVS_ARG_CHECK(images[i], "images[%d] is NULL", i);
-    GMVS_ARG_CHECK(precision == GMVS_PREC_FP32, "unsupported precision %d", precision);
+    GMVS_ARG_CHECK(valid_precision(precision), "unsupported precision %d", precision);
     GMVS_ARG_CHECK(n >= 0, "n must be >= 0");
     if (n == 0) return GMVS_OK;
     GMVS_ARG_CHECK(xy && planes && costs, "xy/planes/costs is NULL");
@@ -1473,7 +1525,7 @@ extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
         return GMVS_ERR_UNSUPPORTED;
     }
     const int h = cameras[0].height, w = cameras[0].width;
-    const PMLayout L = pm_layout(h, w, num_images, false);
+    const PMLayout L = pm_layout(h, w, num_images, false, precision);
     GMVS_ARG_CHECK(workspace, "workspace is NULL");
     if (workspace_bytes < L.total) {
         set_error("workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
@@ -1481,12 +1533,15 @@ extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,
     }
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     char* ws = reinterpret_cast<char*>(workspace);
-    if (int rc = setup_views(num_images, cameras, images, nullptr, ws, L, st)) return rc;
-    PMConst P = make_const(num_images, cameras, images[0],
-                           reinterpret_cast<const ViewDesc*>(ws + L.views), *params, radius);
-    P.quads = reinterpret_cast<const float4*>(ws + L.quads);
-    GMVS_LAUNCH(K_COST_BATCH, st, (pm_cost_batch<kNT, kLanes>), dim3((n * kLanes + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, n, xy,
-                       reinterpret_cast<const float4*>(planes), costs);
-    GMVS_LAUNCH_CHECK();
-    return GMVS_OK;
+    switch (precision) {
+        case GMVS_PREC_FP16:
+            return cost_batch<PREC_F16>(num_images, cameras, images, *params, radius, n, xy, planes,
+                                        costs, ws, L, st);
+        case GMVS_PREC_BF16:
+            return cost_batch<PREC_BF16>(num_images, cameras, images, *params, radius, n, xy, planes,
+                                         costs, ws, L, st);
+        default:
+            return cost_batch<PREC_F32>(num_images, cameras, images, *params, radius, n, xy, planes,
+                                        costs, ws, L, st);
+    }
 }

@@ -39,7 +39,8 @@ class StageOutput:
 class MultiViewPipeline:
     def __init__(self, images: torch.Tensor, cameras: Sequence[S.Camera],
                  pairs: Sequence[Sequence[int]], labels: Optional[Sequence[np.ndarray]] = None,
-                 params: Optional[S.PatchMatchParams] = None, prior_threads: int = 8):
+                 params: Optional[S.PatchMatchParams] = None, prior_threads: int = 8,
+                 precision: int = N.PREC_FP32):
         self.dev = S._device()
         self.images = images.to(self.dev, torch.float32).contiguous()
         self.n, self.h, self.w = (int(x) for x in self.images.shape)
@@ -48,6 +49,7 @@ class MultiViewPipeline:
         self.labels = labels
         self.params = params if params is not None else S.PatchMatchParams()
         self.prior_threads = prior_threads
+        self.precision = precision
         self._ws = S._Workspace()
         self._fuse_ws = S._Workspace()
         for i, p in enumerate(self.pairs):
@@ -80,7 +82,8 @@ class MultiViewPipeline:
             S.run_pm([self.cameras[i]._c for i in ids], [self.images[i] for i in ids],
                      self._cparams(ref, iterations, False), init_mode=N.INIT_RANDOM, geom=False,
                      rng_view=ref, planes=planes[ref], costs=costs[ref], out_depth=out_d[ref],
-                     out_normal=out_n[ref], out_cost=out_c[ref], workspace=self._ws)
+                     out_normal=out_n[ref], out_cost=out_c[ref], workspace=self._ws,
+                     precision=self.precision)
             if planar_prior:
                 host_d[ref].copy_(out_d[ref], non_blocking=True)
                 host_c[ref].copy_(out_c[ref], non_blocking=True)
@@ -118,7 +121,7 @@ class MultiViewPipeline:
                          rng_view=ref, planes=planes[ref], costs=costs[ref],
                          prior_planes=prior_planes[ref], plane_masks=plane_masks[ref],
                          out_depth=out_d[ref], out_normal=out_n[ref], out_cost=out_c[ref],
-                         workspace=self._ws)
+                         workspace=self._ws, precision=self.precision)
             torch.cuda.current_stream().synchronize()  # pinned buffers in `keep` outlive the copies
         return StageOutput(out_d, out_n, out_c)
 
@@ -140,7 +143,7 @@ class MultiViewPipeline:
                      init_mode=N.INIT_DEPTH_NORMAL, geom=True, rng_view=ref, planes=planes,
                      costs=costs, depths=[prev.depth[i] for i in ids], ref_normal=prev.normal[ref],
                      out_depth=out_d[ref], out_normal=out_n[ref], out_cost=out_c[ref],
-                     workspace=self._ws)
+                     workspace=self._ws, precision=self.precision)
         return StageOutput(out_d, out_n, out_c)
 
     def fuse(self, stage: StageOutput, geom_consistent: int = 3,

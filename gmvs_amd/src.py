@@ -190,13 +190,16 @@ def run_pm(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], params: N.
            plane_masks: Optional[torch.Tensor] = None,
            out_depth: Optional[torch.Tensor] = None, out_normal: Optional[torch.Tensor] = None,
            out_cost: Optional[torch.Tensor] = None, workspace: Optional[_Workspace] = None,
-           seed: int = RANDOM_SEED, skip_filter: bool = False) -> None:
-    """Thin call of gmvs_pm_run on device tensors (the current torch stream)."""
+           seed: int = RANDOM_SEED, skip_filter: bool = False,
+           precision: int = N.PREC_FP32) -> None:
+    """Thin call of gmvs_pm_run on device tensors (the current torch stream).
+    precision: N.PREC_FP32 (the plain path), N.PREC_FP16 or N.PREC_BF16
+    (16-bit source samples, FP32 NCC sums)."""
     n = len(images)
     h, w = images[0].shape
     dev = images[0].device
     ws = (workspace or _Workspace()).get(
-        N.lib.gmvs_pm_workspace_bytes(h, w, n, N.PREC_FP32), dev)
+        N.lib.gmvs_pm_workspace_bytes(h, w, n, precision), dev)
     cam_arr = N.camera_array(cams)
     img_arr = N.ptr_array([im.data_ptr() for im in images])
     dep_arr = N.ptr_array([d.data_ptr() if d is not None else None for d in depths]) if depths else None
@@ -215,7 +218,7 @@ def run_pm(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], params: N.
     a.geom = 1 if geom else 0
     a.seed = seed
     a.rng_view = rng_view
-    a.precision = N.PREC_FP32
+    a.precision = precision
     a.skip_filter = 1 if skip_filter else 0
     a.out_depth = out_depth.data_ptr() if out_depth is not None else None
     a.out_normal = out_normal.data_ptr() if out_normal is not None else None

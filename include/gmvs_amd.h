@@ -80,9 +80,13 @@ enum {
     GMVS_INIT_PRIOR = 2         /* RandomInitialization prior branch, lib 1513-1551 */
 };
 
-/* Arithmetic of the image samples and patch products. */
+/* Storage and arithmetic of the source-image samples. In every mode the
+ * homography factors are composed in FP64, tap coordinates are FP32, and the
+ * NCC moments are accumulated in FP32 about the patch-centre intensity. */
 enum {
-    GMVS_PREC_FP32 = 0 /* plain FP32 path */
+    GMVS_PREC_FP32 = 0, /* plain FP32 path: 16-byte bilinear cells, FP32 FMAs */
+    GMVS_PREC_FP16 = 1, /* 8-byte FP16 cells, bilinear as FP16 dot products, FP32 sums */
+    GMVS_PREC_BF16 = 2  /* 8-byte BF16 cells, bilinear as BF16 dot products, FP32 sums */
 };
 
 /* One PatchMatch pass over one reference view. */

@@ -25,7 +25,7 @@ def _ccams(sc, ids):
                           sc.cameras[i].depth_min, sc.cameras[i].depth_max)._c for i in ids]
 
 
-def gpu_cost_batch(sc, ids, xy, planes):
+def gpu_cost_batch(sc, ids, xy, planes, precision=0):
     N, S = _native()
     dev = torch.device("cuda")
     imgs = [sc.images[i].to(dev).contiguous() for i in ids]
@@ -35,12 +35,12 @@ def gpu_cost_batch(sc, ids, xy, planes):
     xy_d = torch.from_numpy(np.ascontiguousarray(xy, np.int32)).to(dev)
     pl_d = torch.from_numpy(np.ascontiguousarray(planes, np.float32)).to(dev)
     out = torch.zeros(len(xy), len(ids) - 1, dtype=torch.float32, device=dev)
-    nbytes = N.lib.gmvs_cost_workspace_bytes(h, w, len(ids), N.PREC_FP32)
+    nbytes = N.lib.gmvs_cost_workspace_bytes(h, w, len(ids), precision)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     cams = N.camera_array(_ccams(sc, ids))
     rc = N.lib.gmvs_cost_batch(len(ids), ctypes.cast(cams, ctypes.POINTER(N.CCamera)),
                                N.ptr_array([im.data_ptr() for im in imgs]), ctypes.byref(p),
-                               N.PREC_FP32, len(xy), ctypes.c_void_p(xy_d.data_ptr()),
+                               precision, len(xy), ctypes.c_void_p(xy_d.data_ptr()),
                                ctypes.c_void_p(pl_d.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                ctypes.c_void_p(ws.data_ptr()), nbytes, None)
     N.check("gmvs_cost_batch", rc)
@@ -77,7 +77,7 @@ def test_cost_batch_matches_fp64(kind):
 
 
 def _pm_gpu(sc, ids, params_kw, init_mode=0, geom=False, depths=None, ref_normal=None,
-            prior=None, state=None, skip_filter=False):
+            prior=None, state=None, skip_filter=False, precision=0):
     N, S = _native()
     dev = torch.device("cuda")
     imgs = [sc.images[i].to(dev).contiguous() for i in ids]
@@ -103,7 +103,8 @@ def _pm_gpu(sc, ids, params_kw, init_mode=0, geom=False, depths=None, ref_normal
         pm = torch.from_numpy(prior[1].astype(np.int32)).to(dev)
     S.run_pm(_ccams(sc, ids), imgs, cp, init_mode=init_mode, geom=geom, rng_view=ids[0],
              planes=planes, costs=costs, depths=dd, ref_normal=rn, prior_planes=pp,
-             plane_masks=pm, out_depth=od, out_normal=on, out_cost=oc, skip_filter=skip_filter)
+             plane_masks=pm, out_depth=od, out_normal=on, out_cost=oc, skip_filter=skip_filter,
+             precision=precision)
     torch.cuda.synchronize()
     return dict(depth=od.cpu().numpy(), normal=on.cpu().numpy(), cost=oc.cpu().numpy(),
                 planes=planes.cpu().numpy())

@@ -103,6 +103,8 @@ EXPORTS = (
     "gmvs_load_problem",
     "gmvs_pm_workspace_bytes",
     "gmvs_pm_run",
+    "gmvs_pm_batch_workspace_bytes",
+    "gmvs_pm_run_batch",
     "gmvs_cost_workspace_bytes",
     "gmvs_cost_batch",
     "gmvs_fuse_workspace_bytes",
@@ -145,6 +147,9 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_pm_workspace_bytes.argtypes = [i32, i32, i32, i32]
     lib.gmvs_pm_workspace_bytes.restype = sz
     lib.gmvs_pm_run.argtypes = [ctypes.POINTER(CPMArgs), vp, sz, vp]
+    lib.gmvs_pm_batch_workspace_bytes.argtypes = [ctypes.POINTER(CPMArgs), i32]
+    lib.gmvs_pm_batch_workspace_bytes.restype = sz
+    lib.gmvs_pm_run_batch.argtypes = [ctypes.POINTER(CPMArgs), i32, vp, sz, vp]
     lib.gmvs_cost_workspace_bytes.argtypes = [i32, i32, i32, i32]
     lib.gmvs_cost_workspace_bytes.restype = sz
     lib.gmvs_cost_batch.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),

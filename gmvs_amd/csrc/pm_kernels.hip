@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "abi_common.h"
@@ -67,7 +68,6 @@ struct PMConst {
     int top_k;
     uint32_t seed, view;
     int init_mode, geom, prior;
-    unsigned long long* work;  // profiling work counters [K_NUM][2] or nullptr
 };
 
 // Adds per-lane work counts (NCCs, geometric terms) to a kernel's counters:
@@ -85,18 +85,99 @@ __device__ inline void count_work(unsigned long long* ctr, uint32_t ncc, uint32_
     }
 }
 
-// Block -> tile with the tiles of one XCD contiguous, so that neighbouring
-// tiles (which gather from the same source-image region) share an L2.
-__device__ inline bool tile_of_block(int ntiles, int& tile) {
-    const int chunk = (ntiles + kXcds - 1) / kXcds;
-    const int b = blockIdx.x;
-    tile = (b % kXcds) * chunk + b / kXcds;
-    return tile < ntiles;
+// ------------------------------------------------------------- batching ---
+// Every pass runs as a batch of problems (reference views, possibly of
+// different scenes and sizes): each kernel of the schedule is launched once
+// for the whole batch over a table of work items (tiles or pixel chunks),
+// ordered problem by problem.
+constexpr int kMaxBatch = 64;
+
+// One problem of a batch: its constants, state and workspace buffers.
+struct PMProblem {
+    PMConst P;
+    float4* planes;         // [H*W] state
+    float* costs;           // [H*W] state
+    uint32_t* sel;          // [H*W] selected-view masks
+    float* ca;              // [9][nv][na] candidate NCC table
+    float4* cplane;         // [8][na] candidate planes
+    uint8_t* cflag;         // [na] candidate flags
+    float4* rtp;            // [5][na] refinement hypotheses
+    uint32_t* rsw;          // [SW_NUM][na] selection record
+    float* out_depth;       // [H*W] or nullptr
+    float* out_normal;      // [H*W*3] or nullptr
+    float* out_cost;        // [H*W] or nullptr
+    int na;                 // active pixels of one colour
+    int skip_filter;
+};
+
+// Work items of one tiling: problem p owns items [start[p], start[p + 1]).
+struct WorkTable {
+    int n;
+    int start[kMaxBatch + 1];
+};
+enum { WT_FULL = 0,  // 8x8-pixel tiles over the image (pm_init)
+       WT_HALF,      // 8x8 tiles over one colour's active pixels (candidates, refine)
+       WT_ACTIVE,    // 256-pixel chunks of active pixels (select, filter)
+       WT_PIXELS,    // 256-pixel chunks of the image (depth/normal, export)
+       WT_NUM };
+
+__device__ inline int problem_of(const WorkTable& t, int item) {
+    int lo = 0, hi = t.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.start[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
-__host__ inline int grid_for_tiles(int ntiles) {
-    const int chunk = (ntiles + kXcds - 1) / kXcds;
-    return chunk * kXcds;
+// Persistent work loop of the heavy kernels: the item space is cut into 8
+// contiguous chunks, one queue per XCD (workgroups are dispatched to the
+// XCDs round-robin, so block b runs on XCD b % 8); a block drains its own
+// XCD's chunk first, so neighbouring tiles (which gather from the same
+// source-image regions) share an L2, then takes what is left in the others.
+// Returns -1 once every queue is exhausted; every block reaches it. The 8
+// counters of a launch start at zero.
+__device__ inline int next_item(uint32_t* __restrict__ ctr, int total, int& q, int& tries,
+                                int& s_item) {
+    __syncthreads();  // the previous item is finished by every thread
+    if (threadIdx.x == 0) {
+        const int chunk = (total + kXcds - 1) / kXcds;
+        int it = -1;
+        while (tries < kXcds) {
+            const int lo = q * chunk;
+            const int hi = min(total, lo + chunk);
+            if (lo < hi) {
+                const int k = (int)atomicAdd(&ctr[q], 1u);
+                if (lo + k < hi) {
+                    it = lo + k;
+                    break;
+                }
+            }
+            q = (q + 1) % kXcds;
+            ++tries;
+        }
+        s_item = it;
+    }
+    __syncthreads();
+    return s_item;
+}
+
+__global__ void pm_zero_words(uint32_t* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Writes one problem record and (for problem 0) the work tables; the
+// records travel as kernel arguments, so the setup is capturable.
+struct WorkTables {
+    WorkTable t[WT_NUM];
+};
+__global__ void pm_setup_problem(PMProblem pr, PMProblem* __restrict__ dst) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = pr;
+}
+__global__ void pm_setup_tables(WorkTables t, WorkTables* __restrict__ dst) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = t;
 }
 
 // ------------------------------------------------------------ setup ---
@@ -181,12 +262,10 @@ __global__ void pm_pack_quads(const float* __restrict__ img, void* __restrict__ 
 
 // ---------------------------------------------------------------- init ---
 template <int NT, int L, int PR>
-__device__ inline void pm_init_body(const PMConst& P, float4* __restrict__ planes,
+__device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restrict__ planes,
                                     float* __restrict__ costs, uint32_t* __restrict__ sel,
                                     float (*s_c)[kBlock], uint32_t& ncc) {
     using TL = Tiling<L>;
-    int tile;
-    if (!tile_of_block(TL::tiles(P.w, P.h), tile)) return;
     const int tx = (P.w + TL::TX - 1) / TL::TX;
     const int slot = threadIdx.x / L, sub = threadIdx.x % L;
     const int px = (tile % tx) * TL::TX + (slot % TL::TX);
@@ -282,18 +361,29 @@ __device__ inline void pm_init_body(const PMConst& P, float4* __restrict__ plane
         planes[c] = pl;
         costs[c] = cost;
         sel[c] = bits;
-        ncc = (uint32_t)P.nv;
+        ncc += (uint32_t)P.nv;
     }
 }
 
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) void pm_init(PMConst P, float4* __restrict__ planes,
-                                                  float* __restrict__ costs,
-                                                  uint32_t* __restrict__ sel) {
+__global__ __launch_bounds__(kBlock) void pm_init(const PMProblem* __restrict__ probs,
+                                                  const WorkTables* __restrict__ tabs,
+                                                  uint32_t* __restrict__ queue,
+                                                  unsigned long long* work) {
     __shared__ float s_c[kMaxViews][kBlock];
+    __shared__ int s_item;
+    const WorkTable& t = tabs->t[WT_FULL];
+    const int total = t.start[t.n];
     uint32_t ncc = 0;
-    pm_init_body<NT, L, PR>(P, planes, costs, sel, s_c, ncc);
-    if (P.work) count_work(P.work + 2 * K_INIT, ncc, 0);
+    int q = blockIdx.x % kXcds, tries = 0;
+    for (;;) {
+        const int item = next_item(queue, total, q, tries, s_item);
+        if (item < 0) break;
+        const int p = problem_of(t, item);
+        const PMProblem& pr = probs[p];
+        pm_init_body<NT, L, PR>(pr.P, item - t.start[p], pr.planes, pr.costs, pr.sel, s_c, ncc);
+    }
+    if (work) count_work(work + 2 * K_INIT, ncc, 0);
 }
 
 // ------------------------------------------------------- half-sweep, A ---
@@ -490,13 +580,11 @@ __device__ inline void candidate_search(const float* __restrict__ cs, int px, in
 }
 
 template <int NT, int L, int PR>
-__device__ inline void pm_candidates_body(const PMConst& P, int color,
+__device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
                                           const float4* __restrict__ planes,
                                           const float* __restrict__ costs, float* __restrict__ ca,
                                           float4* __restrict__ cplane, uint8_t* __restrict__ cflag,
                                           int NA, uint32_t& ncc) {
-    int tile;
-    if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
     const Active a = active_pixel<L>(P, tile, color);
     if (!a.ok) return;
     const bool lead = a.sub == 0;
@@ -505,7 +593,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int color,
     candidate_search(costs, a.px, a.py, P.h, P.w, flags, pos);
     if (lead) {
         cflag[a.aidx] = (uint8_t)flags;
-        ncc = (uint32_t)((__popc(flags) + 1) * P.nv);
+        ncc += (uint32_t)((__popc(flags) + 1) * P.nv);
     }
     Hoist<NT, L> hs;
     hoist_ref<NT, L>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub);
@@ -540,15 +628,24 @@ __device__ inline void pm_candidates_body(const PMConst& P, int color,
 }
 
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) void pm_candidates(PMConst P, int color,
-                                                        const float4* __restrict__ planes,
-                                                        const float* __restrict__ costs,
-                                                        float* __restrict__ ca,
-                                                        float4* __restrict__ cplane,
-                                                        uint8_t* __restrict__ cflag, int NA) {
+__global__ __launch_bounds__(kBlock) void pm_candidates(const PMProblem* __restrict__ probs,
+                                                        const WorkTables* __restrict__ tabs,
+                                                        int color, uint32_t* __restrict__ queue,
+                                                        unsigned long long* work) {
+    __shared__ int s_item;
+    const WorkTable& t = tabs->t[WT_HALF];
+    const int total = t.start[t.n];
     uint32_t ncc = 0;
-    pm_candidates_body<NT, L, PR>(P, color, planes, costs, ca, cplane, cflag, NA, ncc);
-    if (P.work) count_work(P.work + 2 * K_CANDIDATES, ncc, 0);
+    int q = blockIdx.x % kXcds, tries = 0;
+    for (;;) {
+        const int item = next_item(queue, total, q, tries, s_item);
+        if (item < 0) break;
+        const int p = problem_of(t, item);
+        const PMProblem& pr = probs[p];
+        pm_candidates_body<NT, L, PR>(pr.P, item - t.start[p], color, pr.planes, pr.costs, pr.ca,
+                                      pr.cplane, pr.cflag, pr.na, ncc);
+    }
+    if (work) count_work(work + 2 * K_CANDIDATES, ncc, 0);
 }
 
 // ------------------------------------------------------- half-sweep, B ---
@@ -649,14 +746,28 @@ __device__ inline Active active_of_index(const PMConst& P, int color, int aidx) 
     return a;
 }
 
-__global__ __launch_bounds__(kBlock) void pm_select(
-    PMConst P, int color, int iter, float4* __restrict__ planes, float* __restrict__ costs,
-    uint32_t* __restrict__ sel, const float* __restrict__ ca, const float4* __restrict__ cplane,
-    const uint8_t* __restrict__ cflag, float4* __restrict__ rtp, uint32_t* __restrict__ rsw,
-    int NA) {
+// One block per WT_ACTIVE item (256 active pixels of one problem).
+// Dynamic LDS: ViewDesc[nvmax], then the CDF table float[nvmax][kBlock].
+__global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict__ probs,
+                                                   const WorkTables* __restrict__ tabs, int color,
+                                                   int iter, int nvmax, unsigned long long* work) {
     extern __shared__ __align__(16) unsigned char sel_dyn[];
     ViewDesc* s_views = reinterpret_cast<ViewDesc*>(sel_dyn);
-    float* s_cdf = reinterpret_cast<float*>(sel_dyn + sizeof(ViewDesc) * (size_t)P.nv);  // [nv][kBlock]
+    float* s_cdf = reinterpret_cast<float*>(sel_dyn + sizeof(ViewDesc) * (size_t)nvmax);
+    const WorkTable& t = tabs->t[WT_ACTIVE];
+    const int item = blockIdx.x;
+    const int pidx = problem_of(t, item);
+    const PMProblem& pr = probs[pidx];
+    const PMConst& P = pr.P;
+    float4* __restrict__ planes = pr.planes;
+    float* __restrict__ costs = pr.costs;
+    uint32_t* __restrict__ sel = pr.sel;
+    const float* __restrict__ ca = pr.ca;
+    const float4* __restrict__ cplane = pr.cplane;
+    const uint8_t* __restrict__ cflag = pr.cflag;
+    float4* __restrict__ rtp = pr.rtp;
+    uint32_t* __restrict__ rsw = pr.rsw;
+    const int NA = pr.na;
     if (P.geom) {
         const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
         const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
@@ -665,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void pm_select(
         __syncthreads();
     }
     uint32_t geom_count = 0;
-    const int aidx = blockIdx.x * kBlock + threadIdx.x;
+    const int aidx = (item - t.start[pidx]) * kBlock + threadIdx.x;
     const Active a = active_of_index(P, color, aidx);
     if (aidx < NA && a.ok) {
         const int px = a.px, py = a.py;
@@ -880,7 +991,7 @@ __global__ __launch_bounds__(kBlock) void pm_select(
         rsw[(size_t)SW_DCUR * NA + aidx] = __float_as_uint(dcur);
         if (P.geom) geom_count = ((uint32_t)__popc(flags) + 1u) * (uint32_t)__popc(selmask);
     }
-    if (P.work) count_work(P.work + 2 * K_SELECT, 0u, geom_count);
+    if (work) count_work(work + 2 * K_SELECT, 0u, geom_count);
 }
 
 // Pixel `slot` of a pm_refine tile (slot = lane group in the NCC layout).
@@ -917,28 +1028,20 @@ __host__ __device__ inline size_t refine_dyn_bytes(int nv) {
     return refine_res_off(nv) + sizeof(float) * (size_t)kRefine * kK2Pix * nv;
 }
 
+// One tile of pm_refine (every thread of the block calls it: it holds block
+// barriers). s_views holds the problem's view descriptors.
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) void pm_refine(
-    PMConst P, int color, float4* __restrict__ planes, float* __restrict__ costs,
-    const float4* __restrict__ rtp, const uint32_t* __restrict__ rsw, int NA) {
+__device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
+                                      float4* __restrict__ planes, float* __restrict__ costs,
+                                      const float4* __restrict__ rtp,
+                                      const uint32_t* __restrict__ rsw, int NA, RefineSmem<NT>& S,
+                                      const ViewDesc* s_views, uint16_t* units, float* res,
+                                      uint32_t& ncc, uint32_t& geom) {
     static_assert(L == kLanes, "pm_refine is laid out for kLanes-wide groups");
     using B = TapBlock<NT, L>;
     constexpr int NW = RefineSmem<NT>::NW;
-    __shared__ RefineSmem<NT> S;
-    extern __shared__ __align__(16) unsigned char ref_dyn[];
-    ViewDesc* s_views = reinterpret_cast<ViewDesc*>(ref_dyn);
-    uint16_t* units = reinterpret_cast<uint16_t*>(ref_dyn + refine_units_off(P.nv));
-    float* res = reinterpret_cast<float*>(ref_dyn + refine_res_off(P.nv));
-    int tile;
-    if (!tile_of_block(Tiling<L>::tiles(P.wh, P.h), tile)) return;
     const RefCam& rc = P.rc;
     const int g = threadIdx.x / L, sub = threadIdx.x % L;
-    {
-        const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
-        for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
-    }
     // the tile's refinement hypotheses
     for (int i = threadIdx.x; i < kK2Pix * kRefine; i += kBlock) {
         const int sl = i % kK2Pix, f = i / kK2Pix;
@@ -982,7 +1085,6 @@ __global__ __launch_bounds__(kBlock) void pm_refine(
     }
     __syncthreads();
 
-    uint32_t ncc = 0, geom = 0;
     const int nunits = S.nunits;
     for (int u = g; u < nunits; u += kK2Pix) {
         const uint32_t code = units[u];
@@ -1083,13 +1185,53 @@ __global__ __launch_bounds__(kBlock) void pm_refine(
             }
         }
     }
-    if (P.work) count_work(P.work + 2 * K_REFINE, ncc, geom);
+}
+
+// Persistent over WT_HALF tiles. Dynamic LDS: ViewDesc[nvmax], uint16
+// units[64 * nvmax] (slot | view << 6), float results[kRefine][64][nvmax].
+template <int NT, int L, int PR>
+__global__ __launch_bounds__(kBlock) void pm_refine(const PMProblem* __restrict__ probs,
+                                                   const WorkTables* __restrict__ tabs, int color,
+                                                   int nvmax, uint32_t* __restrict__ queue,
+                                                   unsigned long long* work) {
+    __shared__ RefineSmem<NT> S;
+    __shared__ int s_item;
+    extern __shared__ __align__(16) unsigned char ref_dyn[];
+    ViewDesc* s_views = reinterpret_cast<ViewDesc*>(ref_dyn);
+    uint16_t* units = reinterpret_cast<uint16_t*>(ref_dyn + refine_units_off(nvmax));
+    float* res = reinterpret_cast<float*>(ref_dyn + refine_res_off(nvmax));
+    const WorkTable& t = tabs->t[WT_HALF];
+    const int total = t.start[t.n];
+    uint32_t ncc = 0, geom = 0;
+    int q = blockIdx.x % kXcds, tries = 0, loaded = -1;
+    for (;;) {
+        const int item = next_item(queue, total, q, tries, s_item);
+        if (item < 0) break;
+        const int p = problem_of(t, item);
+        const PMProblem& pr = probs[p];
+        if (p != loaded) {  // block-uniform; the body's first barrier orders it
+            const int n = pr.P.nv * (int)(sizeof(ViewDesc) / 4);
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(pr.P.views);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
+            for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+            loaded = p;
+        }
+        pm_refine_body<NT, L, PR>(pr.P, item - t.start[p], color, pr.planes, pr.costs, pr.rtp,
+                                  pr.rsw, pr.na, S, s_views, units, res, ncc, geom);
+    }
+    if (work) count_work(work + 2 * K_REFINE, ncc, geom);
 }
 
 // ------------------------------------------------------ finalisation ---
 // GetDepthandNormal (lib 1655-1674): .w <- depth, normal camera -> world.
-__global__ void pm_depth_normal(PMConst P, float4* __restrict__ planes) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One block per WT_PIXELS item.
+__global__ void pm_depth_normal(const PMProblem* __restrict__ probs,
+                                const WorkTables* __restrict__ tabs) {
+    const WorkTable& t = tabs->t[WT_PIXELS];
+    const int pi = problem_of(t, blockIdx.x);
+    const PMConst& P = probs[pi].P;
+    float4* __restrict__ planes = probs[pi].planes;
+    const int i = (blockIdx.x - t.start[pi]) * blockDim.x + threadIdx.x;
     if (i >= P.h * P.w) return;
     const int py = i / P.w, px = i - py * P.w;
     const RefCam& rc = P.rc;
@@ -1104,10 +1246,17 @@ __global__ void pm_depth_normal(PMConst P, float4* __restrict__ planes) {
 // CheckerboardFilter (src 1165-1281): median of the depth at the pixel and
 // at up to 20 fixed opposite-colour offsets, unless the cost is < 0.001.
 // The median is taken by rank counting over the (at most 21) values.
-__global__ __launch_bounds__(kBlock) void pm_filter(PMConst P, int color,
-                                                    float4* __restrict__ planes,
-                                                    const float* __restrict__ costs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One block per WT_ACTIVE item; problems with skip_filter set are skipped.
+__global__ __launch_bounds__(kBlock) void pm_filter(const PMProblem* __restrict__ probs,
+                                                    const WorkTables* __restrict__ tabs,
+                                                    int color) {
+    const WorkTable& t = tabs->t[WT_ACTIVE];
+    const int p = problem_of(t, blockIdx.x);
+    if (probs[p].skip_filter) return;
+    const PMConst& P = probs[p].P;
+    float4* __restrict__ planes = probs[p].planes;
+    const float* __restrict__ costs = probs[p].costs;
+    const int i = (blockIdx.x - t.start[p]) * blockDim.x + threadIdx.x;
     const int wh = P.wh;
     if (i >= P.h * wh) return;
     const int py = i / wh;
@@ -1174,20 +1323,21 @@ __global__ __launch_bounds__(kBlock) void pm_filter(PMConst P, int color,
     planes[c].w = (n % 2 == 0) ? (lo + hi) / 2.0f : hi;
 }
 
-// export_plane_hypotheses_cost (src 1284-1313).
-__global__ void pm_export(int hw, const float4* __restrict__ planes, const float* __restrict__ costs,
-                          float* __restrict__ depth, float* __restrict__ normal,
-                          float* __restrict__ cost_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= hw) return;
-    const float4 p = planes[i];
-    if (depth) depth[i] = p.w;
-    if (normal) {
-        normal[3 * i + 0] = p.x;
-        normal[3 * i + 1] = p.y;
-        normal[3 * i + 2] = p.z;
+// export_plane_hypotheses_cost (src 1284-1313). One block per WT_PIXELS item.
+__global__ void pm_export(const PMProblem* __restrict__ probs, const WorkTables* __restrict__ tabs) {
+    const WorkTable& t = tabs->t[WT_PIXELS];
+    const int p = problem_of(t, blockIdx.x);
+    const PMProblem& pr = probs[p];
+    const int i = (blockIdx.x - t.start[p]) * blockDim.x + threadIdx.x;
+    if (i >= pr.P.h * pr.P.w) return;
+    const float4 v = pr.planes[i];
+    if (pr.out_depth) pr.out_depth[i] = v.w;
+    if (pr.out_normal) {
+        pr.out_normal[3 * i + 0] = v.x;
+        pr.out_normal[3 * i + 1] = v.y;
+        pr.out_normal[3 * i + 2] = v.z;
     }
-    if (cost_out) cost_out[i] = costs[i];
+    if (pr.out_cost) pr.out_cost[i] = pr.costs[i];
 }
 
 // Parity hook: V NCCs for a list of (pixel, plane) pairs.
@@ -1347,74 +1497,242 @@ PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, co
 
 using namespace gmvs;
 
-// One pass (init, sweeps, depth/normal, filter, export) in cell precision PR.
-template <int PR>
-int run_pass(const gmvs_pm_args& A, char* ws, const PMLayout& L, int radius, hipStream_t st) {
-    const int h = A.cameras[0].height, w = A.cameras[0].width;
-    if (int rc = setup_views<PR>(A.num_images, A.cameras, A.images, A.depths, ws, L, st)) return rc;
+namespace {
 
-    PMConst P = make_const(A.num_images, A.cameras, A.images[0],
-                           reinterpret_cast<const ViewDesc*>(ws + L.views), A.params, radius);
-    P.cells = ws + L.quads;
-    P.init_depth = A.depths ? A.depths[0] : nullptr;
-    P.init_normal = A.ref_normal;
-    P.prior_planes = reinterpret_cast<const float4*>(A.prior_planes);
-    P.plane_masks = A.plane_masks;
-    P.seed = A.seed;
-    P.view = A.rng_view;
-    P.init_mode = A.init_mode;
-    P.geom = A.geom ? 1 : 0;
-    P.prior = A.params.planar_prior ? 1 : 0;
-    P.work = profile_work_counters();
+constexpr int kMaxIterations = 1024;
+constexpr int kQueueSlots = 1 + 4 * kMaxIterations;  // init + (candidates, refine) x 2 colours
 
-    float4* planes = reinterpret_cast<float4*>(A.planes);
-    uint32_t* sel = reinterpret_cast<uint32_t*>(ws + L.sel);
-    float* ca = reinterpret_cast<float*>(ws + L.ca);
-    float4* cplane = reinterpret_cast<float4*>(ws + L.cplane);
-    uint8_t* cflag = reinterpret_cast<uint8_t*>(ws + L.cflag);
-    float4* rtp = reinterpret_cast<float4*>(ws + L.rtp);
-    uint32_t* rsw = reinterpret_cast<uint32_t*>(ws + L.rsw);
-    const int na = h * P.wh;
+// Batch workspace: problem records, work tables and queue counters, then one
+// pm_layout region per problem.
+struct BatchLayout {
+    size_t probs, tabs, queue, total;
+    std::vector<size_t> region;
+};
 
-    const int tiles_full = Tiling<kLanes>::tiles(w, h);
-    GMVS_LAUNCH(K_INIT, st, (pm_init<kNT, kLanes, PR>), dim3(grid_for_tiles(tiles_full)), dim3(kBlock), 0, st, P, planes,
-                       A.costs, sel);
-    GMVS_LAUNCH_CHECK();
-    const int tiles_half = Tiling<kLanes>::tiles(P.wh, h);
-    const size_t sel_lds = sizeof(ViewDesc) * (size_t)P.nv + sizeof(float) * (size_t)P.nv * kBlock;
-    const size_t ref_lds = refine_dyn_bytes(P.nv);
-    const dim3 grid_sel((na + kBlock - 1) / kBlock);
-    const dim3 grid_half(grid_for_tiles(tiles_half));
-    for (int it = 0; it < A.params.max_iterations; ++it) {
-        for (int color = 0; color < 2; ++color) {
-            GMVS_LAUNCH(K_CANDIDATES, st, (pm_candidates<kNT, kLanes, PR>), grid_half, dim3(kBlock), 0, st, P, color, planes,
-                               A.costs, ca, cplane, cflag, na);
-            GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_SELECT, st, pm_select, grid_sel, dim3(kBlock), sel_lds, st, P, color, it,
-                        planes, A.costs, sel, ca, cplane, cflag, rtp, rsw, na);
-            GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_REFINE, st, (pm_refine<kNT, kLanes, PR>), grid_half, dim3(kBlock), ref_lds, st, P,
-                        color, planes, A.costs, rtp, rsw, na);
-            GMVS_LAUNCH_CHECK();
-        }
+BatchLayout batch_layout(const gmvs_pm_args* args, int n) {
+    BatchLayout B;
+    size_t o = 0;
+    B.probs = o;
+    o = align_up(o + sizeof(PMProblem) * n);
+    B.tabs = o;
+    o = align_up(o + sizeof(WorkTables));
+    B.queue = o;
+    o = align_up(o + sizeof(uint32_t) * kXcds * kQueueSlots);
+    for (int i = 0; i < n; ++i) {
+        const gmvs_pm_args& A = args[i];
+        B.region.push_back(o);
+        o += pm_layout(A.cameras[0].height, A.cameras[0].width, A.num_images, true, A.precision).total;
+        o = align_up(o);
     }
-    const int hw = h * w;
-    GMVS_LAUNCH(K_DEPTH_NORMAL, st, pm_depth_normal, dim3((hw + 255) / 256), dim3(256), 0, st, P, planes);
-    GMVS_LAUNCH_CHECK();
-    if (!A.skip_filter) {
-        for (int color = 0; color < 2; ++color) {
-            GMVS_LAUNCH(K_FILTER, st, pm_filter, dim3((na + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P,
-                               color, planes, A.costs);
-            GMVS_LAUNCH_CHECK();
-        }
+    B.total = o;
+    return B;
+}
+
+// Argument checks of one problem (the reference's own preconditions made
+// explicit).
+int check_args(const gmvs_pm_args& A, int* radius) {
+    if (int rc = check_cameras(A.num_images, A.cameras)) return rc;
+    GMVS_ARG_CHECK(A.images, "images is NULL");
+    for (int i = 0; i < A.num_images; ++i) GMVS_ARG_CHECK(A.images[i], "images[%d] is NULL", i);
+    GMVS_ARG_CHECK(A.planes && A.costs, "planes/costs state buffers are required");
+    GMVS_ARG_CHECK(valid_precision(A.precision), "unsupported precision %d", A.precision);
+    GMVS_ARG_CHECK(A.init_mode >= 0 && A.init_mode <= 2, "bad init_mode %d", A.init_mode);
+    GMVS_ARG_CHECK(A.params.max_iterations >= 0 && A.params.max_iterations <= kMaxIterations,
+                   "max_iterations must be in [0, %d]", kMaxIterations);
+    GMVS_ARG_CHECK(A.params.top_k >= 0, "top_k must be >= 0");
+    if (A.init_mode == GMVS_INIT_DEPTH_NORMAL)
+        GMVS_ARG_CHECK(A.depths && A.depths[0] && A.ref_normal,
+                       "INIT_DEPTH_NORMAL needs depths[0] and ref_normal");
+    if (A.geom) {
+        GMVS_ARG_CHECK(A.depths, "the geometric term needs source depth maps");
+        for (int i = 1; i < A.num_images; ++i) GMVS_ARG_CHECK(A.depths[i], "depths[%d] is NULL", i);
     }
-    if (A.out_depth || A.out_normal || A.out_cost) {
-        GMVS_LAUNCH(K_EXPORT, st, pm_export, dim3((hw + 255) / 256), dim3(256), 0, st, hw, planes, A.costs,
-                           A.out_depth, A.out_normal, A.out_cost);
-        GMVS_LAUNCH_CHECK();
+    if (A.params.planar_prior)
+        GMVS_ARG_CHECK(A.prior_planes && A.plane_masks, "planar_prior needs prior planes and masks");
+    const int nt = taps_per_axis(A.params, radius);
+    if (nt != 6) {
+        set_error("patch geometry (patch_size=%d, radius_increment=%d) gives %d taps per axis; "
+                  "this build is specialised for 6 (the reference defaults 11/2)",
+                  A.params.patch_size, A.params.radius_increment, nt);
+        return GMVS_ERR_UNSUPPORTED;
     }
     return GMVS_OK;
 }
+
+// Resident blocks of a persistent kernel on the current device (all CUs x
+// the occupancy at this block size and dynamic LDS), cached per kernel.
+int persistent_grid(const void* kernel, size_t dyn_lds, int items) {
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const auto& e : cache)
+            if (e.first.first == kernel && e.first.second == dyn_lds) per_cu = e.second;
+    }
+    if (per_cu == 0) {
+        int bl = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bl, kernel, kBlock, dyn_lds) != hipSuccess ||
+            bl <= 0)
+            bl = 1;
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        per_cu = bl * cus;
+        std::lock_guard<std::mutex> lk(mu);
+        cache.push_back({{kernel, dyn_lds}, per_cu});
+    }
+    return std::max(1, std::min(per_cu, items));
+}
+
+// One batch of passes (init, sweeps, depth/normal, filter, export), every
+// kernel launched once for all problems, in cell precision PR.
+template <int PR>
+int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, const BatchLayout& B,
+              hipStream_t st) {
+    PMProblem* d_probs = reinterpret_cast<PMProblem*>(ws + B.probs);
+    WorkTables* d_tabs = reinterpret_cast<WorkTables*>(ws + B.tabs);
+    uint32_t* queue = reinterpret_cast<uint32_t*>(ws + B.queue);
+    unsigned long long* work = profile_work_counters();
+    const int iters = args[0].params.max_iterations;
+    const int nq = kXcds * (1 + 4 * iters);
+    GMVS_LAUNCH(K_SETUP_VIEW, st, pm_zero_words, dim3((nq + 255) / 256), dim3(256), 0, st, queue, nq);
+    GMVS_LAUNCH_CHECK();
+
+    WorkTables tabs;
+    std::memset(&tabs, 0, sizeof(tabs));
+    for (int k = 0; k < WT_NUM; ++k) tabs.t[k].n = n;
+    int nvmax = 1;
+    for (int i = 0; i < n; ++i) {
+        const gmvs_pm_args& A = args[i];
+        char* rw = ws + B.region[i];
+        const int h = A.cameras[0].height, w = A.cameras[0].width;
+        const PMLayout L = pm_layout(h, w, A.num_images, true, A.precision);
+        if (int rc = setup_views<PR>(A.num_images, A.cameras, A.images, A.depths, rw, L, st)) return rc;
+        PMProblem pr;
+        std::memset(&pr, 0, sizeof(pr));
+        PMConst& P = pr.P;
+        P = make_const(A.num_images, A.cameras, A.images[0],
+                       reinterpret_cast<const ViewDesc*>(rw + L.views), A.params, radius[i]);
+        P.cells = rw + L.quads;
+        P.init_depth = A.depths ? A.depths[0] : nullptr;
+        P.init_normal = A.ref_normal;
+        P.prior_planes = reinterpret_cast<const float4*>(A.prior_planes);
+        P.plane_masks = A.plane_masks;
+        P.seed = A.seed;
+        P.view = A.rng_view;
+        P.init_mode = A.init_mode;
+        P.geom = A.geom ? 1 : 0;
+        P.prior = A.params.planar_prior ? 1 : 0;
+        pr.planes = reinterpret_cast<float4*>(A.planes);
+        pr.costs = A.costs;
+        pr.sel = reinterpret_cast<uint32_t*>(rw + L.sel);
+        pr.ca = reinterpret_cast<float*>(rw + L.ca);
+        pr.cplane = reinterpret_cast<float4*>(rw + L.cplane);
+        pr.cflag = reinterpret_cast<uint8_t*>(rw + L.cflag);
+        pr.rtp = reinterpret_cast<float4*>(rw + L.rtp);
+        pr.rsw = reinterpret_cast<uint32_t*>(rw + L.rsw);
+        pr.out_depth = A.out_depth;
+        pr.out_normal = A.out_normal;
+        pr.out_cost = A.out_cost;
+        pr.na = h * P.wh;
+        pr.skip_filter = A.skip_filter ? 1 : 0;
+        GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_problem, dim3(1), dim3(64), 0, st, pr, d_probs + i);
+        GMVS_LAUNCH_CHECK();
+        nvmax = std::max(nvmax, P.nv);
+        const int counts[WT_NUM] = {Tiling<kLanes>::tiles(w, h), Tiling<kLanes>::tiles(P.wh, h),
+                                    (pr.na + kBlock - 1) / kBlock, (h * w + kBlock - 1) / kBlock};
+        for (int k = 0; k < WT_NUM; ++k) tabs.t[k].start[i + 1] = tabs.t[k].start[i] + counts[k];
+    }
+    GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_tables, dim3(1), dim3(64), 0, st, tabs, d_tabs);
+    GMVS_LAUNCH_CHECK();
+
+    const int n_full = tabs.t[WT_FULL].start[n], n_half = tabs.t[WT_HALF].start[n];
+    const int n_act = tabs.t[WT_ACTIVE].start[n], n_pix = tabs.t[WT_PIXELS].start[n];
+    const size_t sel_lds = sizeof(ViewDesc) * (size_t)nvmax + sizeof(float) * (size_t)nvmax * kBlock;
+    const size_t ref_lds = refine_dyn_bytes(nvmax);
+    auto k_init = pm_init<kNT, kLanes, PR>;
+    auto k_cand = pm_candidates<kNT, kLanes, PR>;
+    auto k_ref = pm_refine<kNT, kLanes, PR>;
+    const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), 0, n_full));
+    const dim3 g_cand(persistent_grid(reinterpret_cast<const void*>(k_cand), 0, n_half));
+    const dim3 g_ref(persistent_grid(reinterpret_cast<const void*>(k_ref), ref_lds, n_half));
+
+    GMVS_LAUNCH(K_INIT, st, k_init, g_init, dim3(kBlock), 0, st, d_probs, d_tabs, queue, work);
+    GMVS_LAUNCH_CHECK();
+    for (int it = 0; it < iters; ++it) {
+        for (int color = 0; color < 2; ++color) {
+            const int slot = 1 + 2 * (2 * it + color);
+            GMVS_LAUNCH(K_CANDIDATES, st, k_cand, g_cand, dim3(kBlock), 0, st, d_probs, d_tabs, color,
+                        queue + kXcds * slot, work);
+            GMVS_LAUNCH_CHECK();
+            GMVS_LAUNCH(K_SELECT, st, pm_select, dim3(n_act), dim3(kBlock), sel_lds, st, d_probs, d_tabs,
+                        color, it, nvmax, work);
+            GMVS_LAUNCH_CHECK();
+            GMVS_LAUNCH(K_REFINE, st, k_ref, g_ref, dim3(kBlock), ref_lds, st, d_probs, d_tabs, color,
+                        nvmax, queue + kXcds * (slot + 1), work);
+            GMVS_LAUNCH_CHECK();
+        }
+    }
+    GMVS_LAUNCH(K_DEPTH_NORMAL, st, pm_depth_normal, dim3(n_pix), dim3(kBlock), 0, st, d_probs, d_tabs);
+    GMVS_LAUNCH_CHECK();
+    for (int color = 0; color < 2; ++color) {
+        GMVS_LAUNCH(K_FILTER, st, pm_filter, dim3(n_act), dim3(kBlock), 0, st, d_probs, d_tabs, color);
+        GMVS_LAUNCH_CHECK();
+    }
+    GMVS_LAUNCH(K_EXPORT, st, pm_export, dim3(n_pix), dim3(kBlock), 0, st, d_probs, d_tabs);
+    GMVS_LAUNCH_CHECK();
+    return GMVS_OK;
+}
+
+int pm_run_batch(const gmvs_pm_args* args, int n, void* workspace, size_t workspace_bytes,
+                 void* stream) {
+    GMVS_ARG_CHECK(args, "args is NULL");
+    GMVS_ARG_CHECK(n >= 1 && n <= kMaxBatch, "a batch holds 1..%d problems (got %d)", kMaxBatch, n);
+    std::vector<int> radius(n, 0);
+    int nvmax = 1;
+    for (int i = 0; i < n; ++i) {
+        if (int rc = check_args(args[i], &radius[i])) return rc;
+        GMVS_ARG_CHECK(args[i].precision == args[0].precision,
+                       "all problems of a batch share one precision (problem %d)", i);
+        GMVS_ARG_CHECK(args[i].params.max_iterations == args[0].params.max_iterations,
+                       "all problems of a batch share max_iterations (problem %d)", i);
+        nvmax = std::max(nvmax, args[i].num_images - 1);
+    }
+    {
+        // pm_refine keeps per-view descriptors, work units and refinement
+        // results in LDS (~1.4 KB per source view on top of the static part)
+        int dev = 0, lds = 0;
+        GMVS_HIP_CHECK(hipGetDevice(&dev));
+        GMVS_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t need = sizeof(RefineSmem<kNT>) + refine_dyn_bytes(nvmax);
+        if (need > (size_t)lds) {
+            set_error("%d source views need %zu bytes of LDS per block; the device allows %d",
+                      nvmax, need, lds);
+            return GMVS_ERR_UNSUPPORTED;
+        }
+    }
+    const BatchLayout B = batch_layout(args, n);
+    GMVS_ARG_CHECK(workspace, "workspace is NULL");
+    if (workspace_bytes < B.total) {
+        set_error("workspace too small: %zu < %zu bytes", workspace_bytes, B.total);
+        return GMVS_ERR_NOSPACE;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char* ws = reinterpret_cast<char*>(workspace);
+    switch (args[0].precision) {
+        case GMVS_PREC_FP16:
+            return run_batch<PREC_F16>(args, n, radius.data(), ws, B, st);
+        case GMVS_PREC_BF16:
+            return run_batch<PREC_BF16>(args, n, radius.data(), ws, B, st);
+        default:
+            return run_batch<PREC_F32>(args, n, radius.data(), ws, B, st);
+    }
+}
+
+}  // namespace
 
 template <int PR>
 int cost_batch(int num_images, const gmvs_camera* cameras, const float* const* images,
@@ -1434,7 +1752,25 @@ int cost_batch(int num_images, const gmvs_camera* cameras, const float* const* i
 extern "C" size_t gmvs_pm_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
                                           int32_t precision) {
     if (height <= 0 || width <= 0 || num_images < 2 || !valid_precision(precision)) return 0;
-    return pm_layout(height, width, num_images, true, precision).total;
+    gmvs_camera cam;
+    std::memset(&cam, 0, sizeof(cam));
+    cam.height = height;
+    cam.width = width;
+    gmvs_pm_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.cameras = &cam;
+    a.num_images = num_images;
+    a.precision = precision;
+    return batch_layout(&a, 1).total;
+}
+
+extern "C" size_t gmvs_pm_batch_workspace_bytes(const gmvs_pm_args* args, int32_t n) {
+    if (!args || n < 1 || n > kMaxBatch) return 0;
+    for (int i = 0; i < n; ++i)
+        if (!args[i].cameras || args[i].num_images < 2 || args[i].cameras[0].height <= 0 ||
+            args[i].cameras[0].width <= 0 || !valid_precision(args[i].precision))
+            return 0;
+    return batch_layout(args, n).total;
 }
 
 extern "C" size_t gmvs_cost_workspace_bytes(int32_t height, int32_t width, int32_t num_images,
@@ -1446,63 +1782,13 @@ extern "C" size_t gmvs_cost_workspace_bytes(int32_t height, int32_t width, int32
 extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t workspace_bytes,
                            void* stream) {
     clear_error();
-    GMVS_ARG_CHECK(args, "args is NULL");
-    const gmvs_pm_args& A = *args;
-    if (int rc = check_cameras(A.num_images, A.cameras)) return rc;
-    GMVS_ARG_CHECK(A.images, "images is NULL");
-    for (int i = 0; i < A.num_images; ++i) GMVS_ARG_CHECK(A.images[i], "images[%d] is NULL", i);
-    GMVS_ARG_CHECK(A.planes && A.costs, "planes/costs state buffers are required");
-    GMVS_ARG_CHECK(valid_precision(A.precision), "unsupported precision %d", A.precision);
-    GMVS_ARG_CHECK(A.init_mode >= 0 && A.init_mode <= 2, "bad init_mode %d", A.init_mode);
-    GMVS_ARG_CHECK(A.params.max_iterations >= 0, "max_iterations must be >= 0");
-    GMVS_ARG_CHECK(A.params.top_k >= 0, "top_k must be >= 0");
-    if (A.init_mode == GMVS_INIT_DEPTH_NORMAL)
-        GMVS_ARG_CHECK(A.depths && A.depths[0] && A.ref_normal,
-                       "INIT_DEPTH_NORMAL needs depths[0] and ref_normal");
-    if (A.geom) {
-        GMVS_ARG_CHECK(A.depths, "the geometric term needs source depth maps");
-        for (int i = 1; i < A.num_images; ++i) GMVS_ARG_CHECK(A.depths[i], "depths[%d] is NULL", i);
-    }
-    if (A.params.planar_prior)
-        GMVS_ARG_CHECK(A.prior_planes && A.plane_masks, "planar_prior needs prior planes and masks");
-    int radius = 0;
-    const int nt = taps_per_axis(A.params, &radius);
-    if (nt != 6) {
-        set_error("patch geometry (patch_size=%d, radius_increment=%d) gives %d taps per axis; "
-                  "this build is specialised for 6 (the reference defaults 11/2)",
-                  A.params.patch_size, A.params.radius_increment, nt);
-        return GMVS_ERR_UNSUPPORTED;
-    }
-    const int h = A.cameras[0].height, w = A.cameras[0].width;
-    const PMLayout L = pm_layout(h, w, A.num_images, true, A.precision);
-    GMVS_ARG_CHECK(workspace, "workspace is NULL");
-    if (workspace_bytes < L.total) {
-        set_error("workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
-        return GMVS_ERR_NOSPACE;
-    }
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    {
-        // pm_refine keeps per-view descriptors, work units and refinement
-        // results in LDS (~1.4 KB per source view on top of the static part)
-        int dev = 0, lds = 0;
-        GMVS_HIP_CHECK(hipGetDevice(&dev));
-        GMVS_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = sizeof(RefineSmem<kNT>) + refine_dyn_bytes(A.num_images - 1);
-        if (need > (size_t)lds) {
-            set_error("%d source views need %zu bytes of LDS per block; the device allows %d",
-                      A.num_images - 1, need, lds);
-            return GMVS_ERR_UNSUPPORTED;
-        }
-    }
-    char* ws = reinterpret_cast<char*>(workspace);
-    switch (A.precision) {
-        case GMVS_PREC_FP16:
-            return run_pass<PREC_F16>(A, ws, L, radius, st);
-        case GMVS_PREC_BF16:
-            return run_pass<PREC_BF16>(A, ws, L, radius, st);
-        default:
-            return run_pass<PREC_F32>(A, ws, L, radius, st);
-    }
+    return pm_run_batch(args, 1, workspace, workspace_bytes, stream);
+}
+
+extern "C" int gmvs_pm_run_batch(const gmvs_pm_args* args, int32_t n, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+    clear_error();
+    return pm_run_batch(args, n, workspace, workspace_bytes, stream);
 }
 
 extern "C" int gmvs_cost_batch(int32_t num_images, const gmvs_camera* cameras,

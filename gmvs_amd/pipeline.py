@@ -10,9 +10,16 @@ gmvs/scripts/launch.py:122-177 without the per-stage file round trips.
 All views stay resident in HBM. GPU work is issued on the current torch
 stream; the host prior construction of a view (support points, Delaunay,
 plane fit) runs on a thread pool as soon as that view's first pass has
-finished on the device, overlapping the first passes of later views. Launch
-submission is serialised on the calling thread, so the per-call workspace is
-reused in stream order.
+finished on the device, overlapping GPU work of other views. Launch
+submission is serialised on the calling thread, so the workspaces are reused
+in stream order.
+
+With batched=True (the default) the views of a stage run as batches
+(gmvs_pm_run_batch: one launch per kernel for the whole batch): the plain
+passes in `groups` groups, each group's prior pass as soon as its priors are
+built (so the host prior construction of one group overlaps the GPU passes
+of the other), and the geometric pass as one batch. Results are identical to
+batched=False, which runs one view per call as the reference does.
 """
 from __future__ import annotations
 
@@ -40,7 +47,7 @@ class MultiViewPipeline:
     def __init__(self, images: torch.Tensor, cameras: Sequence[S.Camera],
                  pairs: Sequence[Sequence[int]], labels: Optional[Sequence[np.ndarray]] = None,
                  params: Optional[S.PatchMatchParams] = None, prior_threads: int = 8,
-                 precision: int = N.PREC_FP32):
+                 precision: int = N.PREC_FP32, batched: bool = True, groups: int = 2):
         self.dev = S._device()
         self.images = images.to(self.dev, torch.float32).contiguous()
         self.n, self.h, self.w = (int(x) for x in self.images.shape)
@@ -50,6 +57,8 @@ class MultiViewPipeline:
         self.params = params if params is not None else S.PatchMatchParams()
         self.prior_threads = prior_threads
         self.precision = precision
+        self.batched = batched
+        self.groups = max(1, min(groups, self.n))
         self._ws = S._Workspace()
         self._fuse_ws = S._Workspace()
         for i, p in enumerate(self.pairs):
@@ -65,8 +74,100 @@ class MultiViewPipeline:
         p.planar_prior = prior
         return p
 
+    def _problem(self, ref: int, iters: int, prior: bool, **kw) -> dict:
+        ids = self._ids(ref)
+        d = dict(cams=[self.cameras[i]._c for i in ids], images=[self.images[i] for i in ids],
+                 params=self._cparams(ref, iters, prior), rng_view=ref, precision=self.precision)
+        d.update(kw)
+        return d
+
+    def _build_prior(self, ref: int, iterations: int, superpixel_filter: bool, host_d, host_c):
+        lab = self.labels[ref] if (self.labels is not None and superpixel_filter) else None
+        cp = self._cparams(ref, iterations, True)
+        pl, mk, ntri = build_prior(self.cameras[ref]._c, host_d[ref].numpy(), host_c[ref].numpy(),
+                                   lab, cp.depth_min, cp.depth_max, lab is not None)
+        return ref, torch.from_numpy(pl).pin_memory(), torch.from_numpy(mk).pin_memory(), ntri
+
+    def _first_stage_batched(self, iterations: int, planar_prior: bool,
+                             superpixel_filter: bool) -> StageOutput:
+        n, h, w, dev = self.n, self.h, self.w, self.dev
+        planes = torch.zeros(n, h, w, 4, device=dev)
+        costs = torch.zeros(n, h, w, device=dev)
+        out_d = torch.zeros(n, h, w, device=dev)
+        out_n = torch.zeros(n, h, w, 3, device=dev)
+        out_c = torch.zeros(n, h, w, device=dev)
+        host_d = torch.empty(n, h, w, pin_memory=True)
+        host_c = torch.empty(n, h, w, pin_memory=True)
+        stream = torch.cuda.current_stream()
+        groups = [list(g) for g in np.array_split(np.arange(n), self.groups) if len(g)]
+        if not hasattr(self, "_ws_groups") or len(self._ws_groups) != len(groups):
+            self._ws_groups = [S._Workspace() for _ in groups]
+        done = []
+        for gi, g in enumerate(groups):
+            S.run_pm_batch([self._problem(r, iterations, False, init_mode=N.INIT_RANDOM, geom=False,
+                                          planes=planes[r], costs=costs[r], out_depth=out_d[r],
+                                          out_normal=out_n[r], out_cost=out_c[r])
+                            for r in g], self._ws_groups[gi], dev)
+            if planar_prior:
+                for r in g:
+                    host_d[r].copy_(out_d[r], non_blocking=True)
+                    host_c[r].copy_(out_c[r], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(stream)
+                done.append(ev)
+        if not planar_prior:
+            return StageOutput(out_d, out_n, out_c)
+
+        prior_planes = torch.empty(n, h, w, 4, device=dev)
+        plane_masks = torch.empty(n, h, w, dtype=torch.int32, device=dev)
+        self.num_triangles = [0] * n
+
+        def build(gi: int, r: int):
+            done[gi].synchronize()
+            return self._build_prior(r, iterations, superpixel_filter, host_d, host_c)
+
+        with ThreadPoolExecutor(max_workers=max(1, self.prior_threads)) as ex:
+            futures = [[ex.submit(build, gi, r) for r in g] for gi, g in enumerate(groups)]
+            keep = []
+            for gi, g in enumerate(groups):  # each group's prior pass once its priors exist
+                for fut in futures[gi]:
+                    r, pl, mk, ntri = fut.result()
+                    self.num_triangles[r] = ntri
+                    prior_planes[r].copy_(pl, non_blocking=True)
+                    plane_masks[r].copy_(mk, non_blocking=True)
+                    keep.append((pl, mk))
+                S.run_pm_batch([self._problem(r, iterations, True, init_mode=N.INIT_PRIOR, geom=False,
+                                              planes=planes[r], costs=costs[r],
+                                              prior_planes=prior_planes[r],
+                                              plane_masks=plane_masks[r], out_depth=out_d[r],
+                                              out_normal=out_n[r], out_cost=out_c[r])
+                                for r in g], self._ws_groups[gi], dev)
+            torch.cuda.current_stream().synchronize()  # pinned buffers in `keep` outlive the copies
+        return StageOutput(out_d, out_n, out_c)
+
+    def _geom_pass_batched(self, prev: StageOutput, iterations: int) -> StageOutput:
+        n, h, w, dev = self.n, self.h, self.w, self.dev
+        out_d = torch.zeros(n, h, w, device=dev)
+        out_n = torch.zeros(n, h, w, 3, device=dev)
+        out_c = torch.zeros(n, h, w, device=dev)
+        planes = torch.empty(n, h, w, 4, device=dev)
+        costs = torch.empty(n, h, w, device=dev)
+        probs = []
+        for r in range(n):
+            ids = self._ids(r)
+            pb = self._problem(r, iterations, False, init_mode=N.INIT_DEPTH_NORMAL, geom=True,
+                               planes=planes[r], costs=costs[r],
+                               depths=[prev.depth[i] for i in ids], ref_normal=prev.normal[r],
+                               out_depth=out_d[r], out_normal=out_n[r], out_cost=out_c[r])
+            pb["params"].geom_consistency = True
+            probs.append(pb)
+        S.run_pm_batch(probs, self._ws, dev)
+        return StageOutput(out_d, out_n, out_c)
+
     def first_stage(self, iterations: int = 8, planar_prior: bool = True,
                     superpixel_filter: bool = True) -> StageOutput:
+        if self.batched:
+            return self._first_stage_batched(iterations, planar_prior, superpixel_filter)
         n, h, w, dev = self.n, self.h, self.w, self.dev
         planes = torch.zeros(n, h, w, 4, device=dev)
         costs = torch.zeros(n, h, w, device=dev)
@@ -129,6 +230,8 @@ class MultiViewPipeline:
         """One geometric-consistency stage: every view re-initialised from the
         previous stage and costed with the forward-backward reprojection
         term against its sources' previous depth maps."""
+        if self.batched:
+            return self._geom_pass_batched(prev, iterations)
         n, h, w, dev = self.n, self.h, self.w, self.dev
         out_d = torch.zeros(n, h, w, device=dev)
         out_n = torch.zeros(n, h, w, 3, device=dev)

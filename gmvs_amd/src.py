@@ -182,24 +182,19 @@ def pm_params_for_problem(params: PatchMatchParams, ref_cam: Camera, num_images:
     return p
 
 
-def run_pm(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], params: N.CParams, *,
-           init_mode: int, geom: bool, rng_view: int, planes: torch.Tensor, costs: torch.Tensor,
-           depths: Optional[Sequence[torch.Tensor]] = None,
-           ref_normal: Optional[torch.Tensor] = None,
-           prior_planes: Optional[torch.Tensor] = None,
-           plane_masks: Optional[torch.Tensor] = None,
-           out_depth: Optional[torch.Tensor] = None, out_normal: Optional[torch.Tensor] = None,
-           out_cost: Optional[torch.Tensor] = None, workspace: Optional[_Workspace] = None,
-           seed: int = RANDOM_SEED, skip_filter: bool = False,
-           precision: int = N.PREC_FP32) -> None:
-    """Thin call of gmvs_pm_run on device tensors (the current torch stream).
-    precision: N.PREC_FP32 (the plain path), N.PREC_FP16 or N.PREC_BF16
-    (16-bit source samples, FP32 NCC sums)."""
+def make_pm_args(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], params: N.CParams, *,
+                 init_mode: int, geom: bool, rng_view: int, planes: torch.Tensor,
+                 costs: torch.Tensor, depths: Optional[Sequence[torch.Tensor]] = None,
+                 ref_normal: Optional[torch.Tensor] = None,
+                 prior_planes: Optional[torch.Tensor] = None,
+                 plane_masks: Optional[torch.Tensor] = None,
+                 out_depth: Optional[torch.Tensor] = None,
+                 out_normal: Optional[torch.Tensor] = None,
+                 out_cost: Optional[torch.Tensor] = None, seed: int = RANDOM_SEED,
+                 skip_filter: bool = False, precision: int = N.PREC_FP32):
+    """One gmvs_pm_args record over device tensors. Returns (args, keep): keep
+    holds the host arrays the record points to and must outlive the call."""
     n = len(images)
-    h, w = images[0].shape
-    dev = images[0].device
-    ws = (workspace or _Workspace()).get(
-        N.lib.gmvs_pm_workspace_bytes(h, w, n, precision), dev)
     cam_arr = N.camera_array(cams)
     img_arr = N.ptr_array([im.data_ptr() for im in images])
     dep_arr = N.ptr_array([d.data_ptr() if d is not None else None for d in depths]) if depths else None
@@ -223,6 +218,38 @@ def run_pm(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], params: N.
     a.out_depth = out_depth.data_ptr() if out_depth is not None else None
     a.out_normal = out_normal.data_ptr() if out_normal is not None else None
     a.out_cost = out_cost.data_ptr() if out_cost is not None else None
+    return a, (cam_arr, img_arr, dep_arr)
+
+
+def run_pm_batch(problems: Sequence[dict], workspace: Optional[_Workspace] = None,
+                 device=None) -> None:
+    """gmvs_pm_run_batch over problems given as make_pm_args keyword dicts: one
+    schedule, each kernel launched once for all of them (results identical to
+    running them one by one). Asynchronous on the current torch stream."""
+    made = [make_pm_args(**p) for p in problems]
+    arr = (N.CPMArgs * len(made))()
+    for i, (a, _) in enumerate(made):
+        arr[i] = a
+    dev = device if device is not None else problems[0]["images"][0].device
+    nbytes = N.lib.gmvs_pm_batch_workspace_bytes(arr, len(made))
+    if nbytes == 0:
+        raise ValueError("gmvs_pm_batch_workspace_bytes: invalid batch")
+    ws = (workspace or _Workspace()).get(nbytes, dev)
+    N.check("gmvs_pm_run_batch", N.lib.gmvs_pm_run_batch(arr, len(made), ctypes.c_void_p(ws.data_ptr()),
+                                                         ws.numel(), _stream()))
+
+
+def run_pm(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], params: N.CParams, *,
+           workspace: Optional[_Workspace] = None, **kw) -> None:
+    """Thin call of gmvs_pm_run on device tensors (the current torch stream);
+    keywords as make_pm_args. precision: N.PREC_FP32 (the plain path),
+    N.PREC_FP16 or N.PREC_BF16 (16-bit source samples, FP32 NCC sums)."""
+    n = len(images)
+    h, w = images[0].shape
+    prec = kw.get("precision", N.PREC_FP32)
+    ws = (workspace or _Workspace()).get(N.lib.gmvs_pm_workspace_bytes(h, w, n, prec),
+                                         images[0].device)
+    a, keep = make_pm_args(cams, images, params, **kw)
     N.check("gmvs_pm_run", N.lib.gmvs_pm_run(ctypes.byref(a), ctypes.c_void_p(ws.data_ptr()),
                                              ws.numel(), _stream()))
 

@@ -139,6 +139,18 @@ size_t gmvs_pm_workspace_bytes(int32_t height, int32_t width, int32_t num_images
  * (gmvs/lib/src/patch_match_kernel.cu:1711-1810). Asynchronous on `stream`. */
 int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Batch of passes: n problems (reference views of one or several scenes,
+ * sizes may differ) run as one schedule, every kernel launched once for the
+ * whole batch over a table of pixel tiles pulled from per-XCD work queues.
+ * Problems share precision and max_iterations; 1 <= n <= 64. Results are
+ * bit-identical to n separate gmvs_pm_run calls. No counterpart in the
+ * reference, which runs one view per call (PatchMatcher::run_patch_match,
+ * gmvs/src/patch_match.cu:1353-1519). The launch sequence allocates nothing
+ * and can be captured in a HIP graph. */
+size_t gmvs_pm_batch_workspace_bytes(const gmvs_pm_args* args, int32_t n);
+int gmvs_pm_run_batch(const gmvs_pm_args* args, int32_t n, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
 /* Parity hook: V bilateral NCCs for n (pixel, plane) pairs -- the batched form
  * of ComputeMultiViewCostVector (gmvs/lib/src/patch_match_kernel.cu:571-588,
  * gmvs/src/patch_match.cu:503-522). xy: device [n*2] int32, planes: device

@@ -194,3 +194,43 @@ def test_pm_prior_pass_matches_oracle():
           f"triangles = {ntri}")
     assert agree >= 0.97
     assert agree >= agree32 - 0.01
+
+
+@pytest.mark.parametrize("mode", ["random", "depth_normal", "prior"])
+def test_pm_init_only_matches_oracle(mode):
+    """Initialisation alone (max_iterations = 0: init, depth/normal, filter):
+    no Monte-Carlo decisions, so the agreement isolates the three init
+    branches (lib 1467-1577, src 1046-1082)."""
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    cams = T.oracle_cams(sc, ids)
+    imgs = [sc.images[i].numpy() for i in ids]
+    kw = _params(sc, iters=0)
+    if mode == "random":
+        g = _pm_gpu(sc, ids, kw)
+        o = O.pm_run(cams, imgs, O.make_params(**kw), precision=64, rng_view=0)
+    elif mode == "depth_normal":
+        rng = np.random.default_rng(6)
+        depths = [sc.depth[i].numpy() * (1 + 0.01 * rng.standard_normal(sc.depth[i].shape)).astype(np.float32)
+                  for i in ids]
+        normal = sc.normal[0].numpy()
+        g = _pm_gpu(sc, ids, kw, init_mode=1, depths=depths, ref_normal=normal)
+        o = O.pm_run(cams, imgs, O.make_params(**kw), precision=64, init_mode=1, depths=depths,
+                     ref_normal=normal, rng_view=0)
+    else:
+        o1 = O.pm_run(cams, imgs, O.make_params(**_params(sc, iters=2)), precision=64, rng_view=0)
+        from gmvs_amd.lib.build import planar_initer as PI
+
+        prior_planes, masks, ntri = PI.build_prior(_ccams(sc, [0])[0], o1["depth"], o1["cost"], None,
+                                                   kw["depth_min"], kw["depth_max"])
+        kw2 = dict(kw, planar_prior=True)
+        g = _pm_gpu(sc, ids, kw2, init_mode=2, prior=(prior_planes, masks),
+                    state=(o1["planes"].copy(), o1["cost"].copy()))
+        o = O.pm_run(cams, imgs, O.make_params(**kw2), precision=64, init_mode=2,
+                     prior_planes=prior_planes, plane_masks=masks.astype(np.uint32),
+                     planes=o1["planes"], costs=o1["cost"], rng_view=0)
+    agree = T.depth_normal_agreement(g["depth"], g["normal"], o["depth"], o["normal"])
+    cerr = np.abs(g["cost"] - o["cost"])
+    print(f"init {mode}: agreement vs FP64 oracle = {agree:.5f}; cost |err| p99 = "
+          f"{np.quantile(cerr, 0.99):.2e}")
+    assert agree >= 0.999

@@ -9,9 +9,12 @@ fusion of all views (>= 3 consistent views). Inputs are resident in HBM
 before the timed region starts.
 
 Prints one JSON line: value = pixel-hypotheses per second (Mpix-hyp/s; a
-pixel-hypothesis is one plane costed against all V source views), plus the
-fused Mpts/s of the 49-view fusion workload, the roofline of the dominant
-kernel from live HIP-event timing, and the host-CPU baseline.
+pixel-hypothesis is one plane costed against all V source views) of the FP32
+path with each stage's views batched into one schedule (bit-identical to
+running them one by one), plus the same workload view by view in FP32 (the
+plain path) and batched with FP16/BF16 samples, the fused Mpts/s of the
+49-view fusion workload, the roofline of the dominant kernel from live
+HIP-event timing, and the host-CPU baseline.
 """
 from __future__ import annotations
 
@@ -266,11 +269,15 @@ def main() -> int:
     log(f"NCC evaluations per step {total_ncc // args.steps}, "
         f"{total_ncc / (elapsed) / 1e9:.2f} G NCC/s wall, kernel time {total_kernel_ms / args.steps:.1f} ms/step")
 
-    # the same workload with 16-bit source samples (FP32 NCC sums)
+    # The same workload run view by view in FP32 (the reference's structure:
+    # one view per call; the plain path the batched variants are measured
+    # against) and batched with 16-bit source samples (FP32 NCC sums).
+    specs = [("fp32_per_view", N.PREC_FP32, False)]
+    specs += [(f"{v}_batched", N.PRECISIONS[v], True) for v in args.variants.split(",") if v]
     variants = {}
-    for name in [v for v in args.variants.split(",") if v]:
+    for name, prec, batched in specs:
         vp = MultiViewPipeline(sc.images, cams, pairs, labels=labels, prior_threads=8,
-                               precision=N.PRECISIONS[name])
+                               precision=prec, batched=batched)
         for i in range(args.warmup):
             step(vp)
         torch.cuda.synchronize()
@@ -282,9 +289,14 @@ def main() -> int:
         vval = ph_step * args.steps / vt / 1e6
         vfused = int((vfd > 0).sum().item())
         variants[name] = {"value": round(vval, 2), "ms_per_step": round(vt / args.steps * 1e3, 2),
-                          "speedup_vs_fp32": round(vval / value, 3), "fused_points": vfused}
+                          "fused_points": vfused}
         log(f"{name}: {variants[name]}")
         del vp
+    plain = variants.get("fp32_per_view", {}).get("value")
+    if plain:
+        for v in variants.values():
+            v["speedup_vs_plain_fp32"] = round(v["value"] / plain, 3)
+        log(f"fp32_batched: speedup vs plain fp32 {value / plain:.3f}")
 
     fusion = None
     if not args.no_fusion:
@@ -326,12 +338,13 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "acmp_10v_1600x1200_prior_geom_fusion", "views": n,
+        "config": {"workload": "acmp_10v_1600x1200_prior_geom_fusion", "batched": True,
+                   "plain_fp32_per_view": plain, "views": n,
                    "height": h, "width": w, "sources": 9, "iterations": 8, "geom_iterations": 2,
                    "superpixels": 2000, "planes": 24, "pixel_hypotheses_per_step": ph_step,
                    "fused_points": fused},
         "fusion": fusion,
-        "precision_variants": variants,
+        "variants": variants,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

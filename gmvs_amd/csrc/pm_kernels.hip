@@ -589,7 +589,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     if (!a.ok) return;
     const bool lead = a.sub == 0;
     uint32_t flags;
-    int pos[8];
+    int pos[9];
     candidate_search(costs, a.px, a.py, P.h, P.w, flags, pos);
     if (lead) {
         cflag[a.aidx] = (uint8_t)flags;
@@ -597,32 +597,29 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     }
     Hoist<NT, L> hs;
     hoist_ref<NT, L>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub);
-    for (int k = 0; k < 8; ++k) {
-        if (!((flags >> k) & 1u)) continue;
-        const float4 pl = planes[pos[k]];
-        if (lead) cplane[(size_t)k * NA + a.aidx] = pl;
-        float m[3];
-        plane_m(P.rc, pl, m);
-        for (int v = 0; v < P.nv; ++v) {
-            const ViewDesc& vd = P.views[v];
-            float H[9];
-            homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
-            if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
+    // hypotheses 0-7: the flagged candidates; 8: the pixel's own current
+    // plane against every view (the selection weights the selected ones)
+    pos[8] = a.py * P.w + a.px;
+    const uint32_t todo = flags | (1u << 8);
+    if (lead)
+        for (uint32_t mm = flags; mm; mm &= mm - 1u) {
+            const int k = __ffs(mm) - 1;
+            cplane[(size_t)k * NA + a.aidx] = planes[pos[k]];
         }
-    }
-    // row 8: the pixel's own current hypothesis against every view (the
-    // select/refine kernel weights the selected ones)
-    {
-        const float4 pl = planes[a.py * P.w + a.px];
-        float m[3];
-        plane_m(P.rc, pl, m);
-        for (int v = 0; v < P.nv; ++v) {
-            const ViewDesc& vd = P.views[v];
+    // views outermost: the consecutive NCCs of a view gather from one source
+    // image region (the hypotheses are neighbours' planes), which keeps the
+    // wave's gathers in L1
+    for (int v = 0; v < P.nv; ++v) {
+        const ViewDesc& vd = P.views[v];
+        const QuadSrc src = quad_src<PR>(vd.cells, P.h * P.w, 0);
+        for (uint32_t mm = todo; mm; mm &= mm - 1u) {
+            const int k = __ffs(mm) - 1;
+            float m[3];
+            plane_m(P.rc, planes[pos[k]], m);
             float H[9];
             homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
-            if (lead) ca[((size_t)8 * P.nv + v) * NA + a.aidx] = cv;
+            const float cv = ncc_eval<NT, L, PR>(hs, H, src, P.h, P.w);
+            if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
     }
 }
@@ -1066,22 +1063,23 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
         }
     }
     __syncthreads();
-    // work units (slot, view), slot-major: wave 0 scans the view counts
+    // work units (slot, view), view-major: the groups working at the same
+    // time gather from one source view for neighbouring pixels. Wave 0
+    // (one lane per slot) ranks each pixel among the tile's pixels that
+    // selected view v with a ballot.
     if (threadIdx.x < kK2Pix) {
         const int sl = threadIdx.x;
         const Active a = active_slot(P, tile, color, sl);
         const uint32_t mask = a.ok ? rsw[(size_t)SW_SELMASK * NA + a.aidx] : 0u;
-        const int n = __popc(mask);
-        int incl = n;
-#pragma unroll
-        for (int off = 1; off < kK2Pix; off <<= 1) {
-            const int t = __shfl_up(incl, off);
-            if (sl >= off) incl += t;
+        const uint64_t below = (sl == 0) ? 0ull : (~0ull >> (64 - sl));
+        int base = 0;
+        for (int v = 0; v < P.nv; ++v) {
+            const bool on = (mask >> v) & 1u;
+            const uint64_t b = __ballot(on);
+            if (on) units[base + __popcll(b & below)] = (uint16_t)(sl | (v << 6));
+            base += __popcll(b);
         }
-        int o = incl - n;
-        for (uint32_t mm = mask; mm; mm &= mm - 1u)
-            units[o++] = (uint16_t)(sl | ((__ffs(mm) - 1) << 6));
-        if (sl == kK2Pix - 1) S.nunits = incl;
+        if (sl == 0) S.nunits = base;
     }
     __syncthreads();
 

@@ -176,9 +176,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
-    ap.add_argument("--variants", default="fp16,bf16",
-                    help="16-bit sample precisions also timed on the same workload "
-                         "(comma list of fp16, bf16; empty for none)")
+    ap.add_argument("--variants", default="plain,fp16,bf16",
+                    help="other runs of the same workload: plain (FP32, one view per call), "
+                         "fp16 / bf16 (batched, 16-bit samples); comma list, empty for none")
     ap.add_argument("--summarize-profile", metavar="DIR", default=None,
                     help="print a summary of rocprofv3 CSV output in DIR and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -272,8 +272,12 @@ def main() -> int:
     # The same workload run view by view in FP32 (the reference's structure:
     # one view per call; the plain path the batched variants are measured
     # against) and batched with 16-bit source samples (FP32 NCC sums).
-    specs = [("fp32_per_view", N.PREC_FP32, False)]
-    specs += [(f"{v}_batched", N.PRECISIONS[v], True) for v in args.variants.split(",") if v]
+    specs = []
+    for v in [v for v in args.variants.split(",") if v]:
+        if v == "plain":
+            specs.append(("fp32_per_view", N.PREC_FP32, False))
+        else:
+            specs.append((f"{v}_batched", N.PRECISIONS[v], True))
     variants = {}
     for name, prec, batched in specs:
         vp = MultiViewPipeline(sc.images, cams, pairs, labels=labels, prior_threads=8,

@@ -92,6 +92,7 @@ class CPMArgs(ctypes.Structure):
         ("out_depth", ctypes.c_void_p),
         ("out_normal", ctypes.c_void_p),
         ("out_cost", ctypes.c_void_p),
+        ("tile_cols", ctypes.c_int32),
     ]
 
 

@@ -35,16 +35,16 @@ namespace gmvs {
 constexpr int kBlock = 256;  // threads per block
 constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
 
-// Lane groups of L lanes evaluate one pixel's NCCs together. Per block
-// 256 / L pixels in a TX x TY tile (TX counts active columns in the
-// half-sweep kernels and pixel columns elsewhere).
-template <int L>
-struct Tiling {
-    static constexpr int PIX = kBlock / L;
-    static constexpr int TX = (L == 4) ? 8 : 16;
-    static constexpr int TY = PIX / TX;
-    __host__ __device__ static int tiles(int cols, int rows) {
-        return ((cols + TX - 1) / TX) * ((rows + TY - 1) / TY);
+// Lane groups of kLanes lanes evaluate one pixel's NCCs together: a block
+// of 256 threads holds a tile of 64 pixels, TX x TY with TX = 1 << shift
+// (TX counts active columns in the half-sweep kernels and pixel columns in
+// pm_init). The shape is chosen per problem (gmvs_pm_args.tile_cols).
+struct TileShape {
+    int shift;  // log2 of the tile's columns
+    __host__ __device__ int tx() const { return 1 << shift; }
+    __host__ __device__ int ty() const { return 64 >> shift; }
+    __host__ __device__ int tiles(int cols, int rows) const {
+        return ((cols + tx() - 1) >> shift) * ((rows + ty() - 1) / ty());
     }
 };
 
@@ -68,6 +68,7 @@ struct PMConst {
     int top_k;
     uint32_t seed, view;
     int init_mode, geom, prior;
+    TileShape tile;
 };
 
 // Adds per-lane work counts (NCCs, geometric terms) to a kernel's counters:
@@ -265,11 +266,11 @@ template <int NT, int L, int PR>
 __device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restrict__ planes,
                                     float* __restrict__ costs, uint32_t* __restrict__ sel,
                                     float (*s_c)[kBlock], uint32_t& ncc) {
-    using TL = Tiling<L>;
-    const int tx = (P.w + TL::TX - 1) / TL::TX;
+    const int TX = P.tile.tx(), TY = P.tile.ty();
+    const int tx = (P.w + TX - 1) / TX;
     const int slot = threadIdx.x / L, sub = threadIdx.x % L;
-    const int px = (tile % tx) * TL::TX + (slot % TL::TX);
-    const int py = (tile / tx) * TL::TY + (slot / TL::TX);
+    const int px = (tile % tx) * TX + (slot % TX);
+    const int py = (tile / tx) * TY + (slot / TX);
     if (px >= P.w || py >= P.h) return;
     const int c = py * P.w + px;
     const RefCam& rc = P.rc;
@@ -394,11 +395,11 @@ struct Active {
 
 template <int L>
 __device__ inline Active active_pixel(const PMConst& P, int tile, int color) {
-    using TL = Tiling<L>;
-    const int tx = (P.wh + TL::TX - 1) / TL::TX;
+    const int TX = P.tile.tx(), TY = P.tile.ty();
+    const int tx = (P.wh + TX - 1) / TX;
     const int slot = threadIdx.x / L;
-    const int ax = (tile % tx) * TL::TX + (slot % TL::TX);
-    const int y = (tile / tx) * TL::TY + (slot / TL::TX);
+    const int ax = (tile % tx) * TX + (slot % TX);
+    const int y = (tile / tx) * TY + (slot / TX);
     Active a;
     a.sub = threadIdx.x % L;
     a.py = y;
@@ -584,7 +585,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
                                           const float4* __restrict__ planes,
                                           const float* __restrict__ costs, float* __restrict__ ca,
                                           float4* __restrict__ cplane, uint8_t* __restrict__ cflag,
-                                          int NA, uint32_t& ncc) {
+                                          int NA, float (*s_m)[3][kBlock], uint32_t& ncc) {
     const Active a = active_pixel<L>(P, tile, color);
     if (!a.ok) return;
     const bool lead = a.sub == 0;
@@ -601,10 +602,18 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     // plane against every view (the selection weights the selected ones)
     pos[8] = a.py * P.w + a.px;
     const uint32_t todo = flags | (1u << 8);
-    if (lead)
-        for (uint32_t mm = flags; mm; mm &= mm - 1u) {
-            const int k = __ffs(mm) - 1;
-            cplane[(size_t)k * NA + a.aidx] = planes[pos[k]];
+    // each lane keeps the rank-1 factors m of its pixel's hypotheses in its
+    // own LDS column, so the view loop reads 3 floats per NCC
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        if ((todo >> k) & 1u) {
+            const float4 pl = planes[pos[k]];
+            if (lead && k < 8) cplane[(size_t)k * NA + a.aidx] = pl;
+            float m[3];
+            plane_m(P.rc, pl, m);
+            s_m[k][0][threadIdx.x] = m[0];
+            s_m[k][1][threadIdx.x] = m[1];
+            s_m[k][2][threadIdx.x] = m[2];
         }
     // views outermost: the consecutive NCCs of a view gather from one source
     // image region (the hypotheses are neighbours' planes), which keeps the
@@ -614,8 +623,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
         const QuadSrc src = quad_src<PR>(vd.cells, P.h * P.w, 0);
         for (uint32_t mm = todo; mm; mm &= mm - 1u) {
             const int k = __ffs(mm) - 1;
-            float m[3];
-            plane_m(P.rc, planes[pos[k]], m);
+            const float m[3] = {s_m[k][0][threadIdx.x], s_m[k][1][threadIdx.x], s_m[k][2][threadIdx.x]};
             float H[9];
             homography(vd.A, vd.b, m, H);
             const float cv = ncc_eval<NT, L, PR>(hs, H, src, P.h, P.w);
@@ -630,6 +638,7 @@ __global__ __launch_bounds__(kBlock) void pm_candidates(const PMProblem* __restr
                                                         int color, uint32_t* __restrict__ queue,
                                                         unsigned long long* work) {
     __shared__ int s_item;
+    __shared__ float s_m[9][3][kBlock];
     const WorkTable& t = tabs->t[WT_HALF];
     const int total = t.start[t.n];
     uint32_t ncc = 0;
@@ -640,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void pm_candidates(const PMProblem* __restr
         const int p = problem_of(t, item);
         const PMProblem& pr = probs[p];
         pm_candidates_body<NT, L, PR>(pr.P, item - t.start[p], color, pr.planes, pr.costs, pr.ca,
-                                      pr.cplane, pr.cflag, pr.na, ncc);
+                                      pr.cplane, pr.cflag, pr.na, s_m, ncc);
     }
     if (work) count_work(work + 2 * K_CANDIDATES, ncc, 0);
 }
@@ -993,10 +1002,10 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
 
 // Pixel `slot` of a pm_refine tile (slot = lane group in the NCC layout).
 __device__ inline Active active_slot(const PMConst& P, int tile, int color, int slot) {
-    using TL = Tiling<kLanes>;
-    const int tx = (P.wh + TL::TX - 1) / TL::TX;
-    const int ax = (tile % tx) * TL::TX + (slot % TL::TX);
-    const int y = (tile / tx) * TL::TY + (slot / TL::TX);
+    const int TX = P.tile.tx(), TY = P.tile.ty();
+    const int tx = (P.wh + TX - 1) / TX;
+    const int ax = (tile % tx) * TX + (slot % TX);
+    const int y = (tile / tx) * TY + (slot / TX);
     Active a;
     a.sub = 0;
     a.py = y;
@@ -1526,6 +1535,16 @@ BatchLayout batch_layout(const gmvs_pm_args* args, int n) {
     return B;
 }
 
+// Tile columns of a 64-pixel tile; 0 selects the default 8 x 8.
+int tile_shift(int cols) {
+    switch (cols) {
+        case 4: return 2;
+        case 16: return 4;
+        case 32: return 5;
+        default: return 3;
+    }
+}
+
 // Argument checks of one problem (the reference's own preconditions made
 // explicit).
 int check_args(const gmvs_pm_args& A, int* radius) {
@@ -1538,6 +1557,9 @@ int check_args(const gmvs_pm_args& A, int* radius) {
     GMVS_ARG_CHECK(A.params.max_iterations >= 0 && A.params.max_iterations <= kMaxIterations,
                    "max_iterations must be in [0, %d]", kMaxIterations);
     GMVS_ARG_CHECK(A.params.top_k >= 0, "top_k must be >= 0");
+    GMVS_ARG_CHECK(A.tile_cols == 0 || A.tile_cols == 4 || A.tile_cols == 8 || A.tile_cols == 16 ||
+                       A.tile_cols == 32,
+                   "tile_cols must be 0, 4, 8, 16 or 32 (got %d)", A.tile_cols);
     if (A.init_mode == GMVS_INIT_DEPTH_NORMAL)
         GMVS_ARG_CHECK(A.depths && A.depths[0] && A.ref_normal,
                        "INIT_DEPTH_NORMAL needs depths[0] and ref_normal");
@@ -1624,6 +1646,7 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
         P.init_mode = A.init_mode;
         P.geom = A.geom ? 1 : 0;
         P.prior = A.params.planar_prior ? 1 : 0;
+        P.tile.shift = tile_shift(A.tile_cols);
         pr.planes = reinterpret_cast<float4*>(A.planes);
         pr.costs = A.costs;
         pr.sel = reinterpret_cast<uint32_t*>(rw + L.sel);
@@ -1640,7 +1663,7 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
         GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_problem, dim3(1), dim3(64), 0, st, pr, d_probs + i);
         GMVS_LAUNCH_CHECK();
         nvmax = std::max(nvmax, P.nv);
-        const int counts[WT_NUM] = {Tiling<kLanes>::tiles(w, h), Tiling<kLanes>::tiles(P.wh, h),
+        const int counts[WT_NUM] = {P.tile.tiles(w, h), P.tile.tiles(P.wh, h),
                                     (pr.na + kBlock - 1) / kBlock, (h * w + kBlock - 1) / kBlock};
         for (int k = 0; k < WT_NUM; ++k) tabs.t[k].start[i + 1] = tabs.t[k].start[i] + counts[k];
     }

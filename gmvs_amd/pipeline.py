@@ -47,7 +47,8 @@ class MultiViewPipeline:
     def __init__(self, images: torch.Tensor, cameras: Sequence[S.Camera],
                  pairs: Sequence[Sequence[int]], labels: Optional[Sequence[np.ndarray]] = None,
                  params: Optional[S.PatchMatchParams] = None, prior_threads: int = 8,
-                 precision: int = N.PREC_FP32, batched: bool = True, groups: int = 2):
+                 precision: int = N.PREC_FP32, batched: bool = True, groups: int = 2,
+                 tile_cols: int = 0):
         self.dev = S._device()
         self.images = images.to(self.dev, torch.float32).contiguous()
         self.n, self.h, self.w = (int(x) for x in self.images.shape)
@@ -58,6 +59,7 @@ class MultiViewPipeline:
         self.prior_threads = prior_threads
         self.precision = precision
         self.batched = batched
+        self.tile_cols = tile_cols
         self.groups = max(1, min(groups, self.n))
         self._ws = S._Workspace()
         self._fuse_ws = S._Workspace()
@@ -77,7 +79,8 @@ class MultiViewPipeline:
     def _problem(self, ref: int, iters: int, prior: bool, **kw) -> dict:
         ids = self._ids(ref)
         d = dict(cams=[self.cameras[i]._c for i in ids], images=[self.images[i] for i in ids],
-                 params=self._cparams(ref, iters, prior), rng_view=ref, precision=self.precision)
+                 params=self._cparams(ref, iters, prior), rng_view=ref, precision=self.precision,
+                 tile_cols=self.tile_cols)
         d.update(kw)
         return d
 
@@ -184,7 +187,7 @@ class MultiViewPipeline:
                      self._cparams(ref, iterations, False), init_mode=N.INIT_RANDOM, geom=False,
                      rng_view=ref, planes=planes[ref], costs=costs[ref], out_depth=out_d[ref],
                      out_normal=out_n[ref], out_cost=out_c[ref], workspace=self._ws,
-                     precision=self.precision)
+                     precision=self.precision, tile_cols=self.tile_cols)
             if planar_prior:
                 host_d[ref].copy_(out_d[ref], non_blocking=True)
                 host_c[ref].copy_(out_c[ref], non_blocking=True)
@@ -222,7 +225,7 @@ class MultiViewPipeline:
                          rng_view=ref, planes=planes[ref], costs=costs[ref],
                          prior_planes=prior_planes[ref], plane_masks=plane_masks[ref],
                          out_depth=out_d[ref], out_normal=out_n[ref], out_cost=out_c[ref],
-                         workspace=self._ws, precision=self.precision)
+                         workspace=self._ws, precision=self.precision, tile_cols=self.tile_cols)
             torch.cuda.current_stream().synchronize()  # pinned buffers in `keep` outlive the copies
         return StageOutput(out_d, out_n, out_c)
 
@@ -246,7 +249,7 @@ class MultiViewPipeline:
                      init_mode=N.INIT_DEPTH_NORMAL, geom=True, rng_view=ref, planes=planes,
                      costs=costs, depths=[prev.depth[i] for i in ids], ref_normal=prev.normal[ref],
                      out_depth=out_d[ref], out_normal=out_n[ref], out_cost=out_c[ref],
-                     workspace=self._ws, precision=self.precision)
+                     workspace=self._ws, precision=self.precision, tile_cols=self.tile_cols)
         return StageOutput(out_d, out_n, out_c)
 
     def fuse(self, stage: StageOutput, geom_consistent: int = 3,

@@ -112,6 +112,9 @@ typedef struct gmvs_pm_args {
     float* out_depth;             /* device [H*W] or NULL */
     float* out_normal;            /* device [H*W*3] or NULL */
     float* out_cost;              /* device [H*W] or NULL */
+    int32_t tile_cols;            /* shape of the 64-pixel work tile: columns 4, 8, 16 or 32
+                                     (rows = 64 / cols); 0 = 8 x 8. Results do not depend
+                                     on it (chosen by gmvs_amd.autotune) */
 } gmvs_pm_args;
 
 /* Last error message of the calling thread ("" if none). */

@@ -455,11 +455,20 @@ struct Cell<PREC_BF16> {
     }
 };
 
+// The descriptor of a view's cells. cells and count must be uniform across
+// the wave (they are: one view of one problem at a time); readfirstlane
+// states it, so the descriptor lives in SGPRs even when the compiler cannot
+// prove the uniformity (values loaded from the problem table with vector
+// loads), instead of wrapping every gather in a waterfall loop.
 template <int PR>
 __device__ inline QuadSrc quad_src(const void* cells, uint32_t count, int base) {
+    const uint64_t a = reinterpret_cast<uint64_t>(cells);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    void* ua = reinterpret_cast<void*>(((uint64_t)hi << 32) | lo);
     QuadSrc q;
-    q.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(cells), (short)Cell<PR>::kBytes,
-                                               (int)count, kBufWord3);
+    q.rsrc = __builtin_amdgcn_make_buffer_rsrc(ua, (short)Cell<PR>::kBytes,
+                                               (int)__builtin_amdgcn_readfirstlane(count), kBufWord3);
     q.base = base;
     return q;
 }

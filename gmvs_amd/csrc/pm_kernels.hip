@@ -161,7 +161,10 @@ __device__ inline int next_item(uint32_t* __restrict__ ctr, int total, int& q, i
         s_item = it;
     }
     __syncthreads();
-    return s_item;
+    // block-uniform; readfirstlane tells the compiler so, which keeps the
+    // problem record and the buffer descriptors derived from it in SGPRs
+    // (a value loaded from LDS is otherwise treated as divergent)
+    return __builtin_amdgcn_readfirstlane(s_item);
 }
 
 __global__ void pm_zero_words(uint32_t* __restrict__ p, int n) {

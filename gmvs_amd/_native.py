@@ -28,6 +28,10 @@ PREC_FP16 = 1
 PREC_BF16 = 2
 PRECISIONS = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16}
 
+TUNE_INIT = 0
+TUNE_CANDIDATES = 1
+TUNE_REFINE = 2
+
 
 class CCamera(ctypes.Structure):
     _fields_ = [
@@ -106,6 +110,7 @@ EXPORTS = (
     "gmvs_pm_run",
     "gmvs_pm_batch_workspace_bytes",
     "gmvs_pm_run_batch",
+    "gmvs_set_blocks_per_cu",
     "gmvs_cost_workspace_bytes",
     "gmvs_cost_batch",
     "gmvs_fuse_workspace_bytes",
@@ -151,6 +156,7 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_pm_batch_workspace_bytes.argtypes = [ctypes.POINTER(CPMArgs), i32]
     lib.gmvs_pm_batch_workspace_bytes.restype = sz
     lib.gmvs_pm_run_batch.argtypes = [ctypes.POINTER(CPMArgs), i32, vp, sz, vp]
+    lib.gmvs_set_blocks_per_cu.argtypes = [i32, i32]
     lib.gmvs_cost_workspace_bytes.argtypes = [i32, i32, i32, i32]
     lib.gmvs_cost_workspace_bytes.restype = sz
     lib.gmvs_cost_batch.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cstring>
 #include <mutex>
@@ -1582,9 +1583,14 @@ int check_args(const gmvs_pm_args& A, int* radius) {
     return GMVS_OK;
 }
 
-// Resident blocks of a persistent kernel on the current device (all CUs x
-// the occupancy at this block size and dynamic LDS), cached per kernel.
-int persistent_grid(const void* kernel, size_t dyn_lds, int items) {
+// Blocks per CU of the persistent kernels (0 = as many as fit), set by
+// gmvs_set_blocks_per_cu; the autotuner picks them by timing.
+std::atomic<int> g_blocks_per_cu[K_NUM];
+
+// Blocks of a persistent kernel on the current device: all CUs x the
+// occupancy at this block size and dynamic LDS (cached per kernel), or the
+// tuned blocks per CU when that is lower.
+int persistent_grid(const void* kernel, size_t dyn_lds, int items, int kid) {
     static std::mutex mu;
     static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
     int per_cu = 0;
@@ -1606,6 +1612,14 @@ int persistent_grid(const void* kernel, size_t dyn_lds, int items) {
         per_cu = bl * cus;
         std::lock_guard<std::mutex> lk(mu);
         cache.push_back({{kernel, dyn_lds}, per_cu});
+    }
+    const int tuned = g_blocks_per_cu[kid].load();
+    if (tuned > 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            cus > 0)
+            per_cu = std::min(per_cu, tuned * cus);
     }
     return std::max(1, std::min(per_cu, items));
 }
@@ -1680,9 +1694,9 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
     auto k_init = pm_init<kNT, kLanes, PR>;
     auto k_cand = pm_candidates<kNT, kLanes, PR>;
     auto k_ref = pm_refine<kNT, kLanes, PR>;
-    const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), 0, n_full));
-    const dim3 g_cand(persistent_grid(reinterpret_cast<const void*>(k_cand), 0, n_half));
-    const dim3 g_ref(persistent_grid(reinterpret_cast<const void*>(k_ref), ref_lds, n_half));
+    const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), 0, n_full, K_INIT));
+    const dim3 g_cand(persistent_grid(reinterpret_cast<const void*>(k_cand), 0, n_half, K_CANDIDATES));
+    const dim3 g_ref(persistent_grid(reinterpret_cast<const void*>(k_ref), ref_lds, n_half, K_REFINE));
 
     GMVS_LAUNCH(K_INIT, st, k_init, g_init, dim3(kBlock), 0, st, d_probs, d_tabs, queue, work);
     GMVS_LAUNCH_CHECK();
@@ -1801,6 +1815,19 @@ extern "C" size_t gmvs_cost_workspace_bytes(int32_t height, int32_t width, int32
                                             int32_t precision) {
     if (height <= 0 || width <= 0 || num_images < 2 || !valid_precision(precision)) return 0;
     return pm_layout(height, width, num_images, false, precision).total;
+}
+
+extern "C" int gmvs_set_blocks_per_cu(int32_t kernel, int32_t blocks) {
+    clear_error();
+    GMVS_ARG_CHECK(kernel == GMVS_TUNE_INIT || kernel == GMVS_TUNE_CANDIDATES ||
+                       kernel == GMVS_TUNE_REFINE,
+                   "unknown tunable kernel %d", kernel);
+    GMVS_ARG_CHECK(blocks >= 0 && blocks <= 64, "blocks per CU must be in [0, 64] (got %d)", blocks);
+    const int kid = kernel == GMVS_TUNE_INIT ? K_INIT
+                    : kernel == GMVS_TUNE_CANDIDATES ? K_CANDIDATES
+                                                     : K_REFINE;
+    g_blocks_per_cu[kid].store(blocks);
+    return GMVS_OK;
 }
 
 extern "C" int gmvs_pm_run(const gmvs_pm_args* args, void* workspace, size_t workspace_bytes,

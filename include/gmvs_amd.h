@@ -154,6 +154,17 @@ size_t gmvs_pm_batch_workspace_bytes(const gmvs_pm_args* args, int32_t n);
 int gmvs_pm_run_batch(const gmvs_pm_args* args, int32_t n, void* workspace,
                       size_t workspace_bytes, void* stream);
 
+/* Launch tuning (no counterpart in the reference): blocks per CU of the
+ * persistent PatchMatch kernels, 0 = as many as fit (the default). Fewer
+ * resident waves can win when the gathers contend for the L1; results do
+ * not depend on it. Process-wide; picked by gmvs_amd.autotune. */
+enum {
+    GMVS_TUNE_INIT = 0,       /* pm_init */
+    GMVS_TUNE_CANDIDATES = 1, /* pm_candidates */
+    GMVS_TUNE_REFINE = 2      /* pm_refine */
+};
+int gmvs_set_blocks_per_cu(int32_t kernel, int32_t blocks);
+
 /* Parity hook: V bilateral NCCs for n (pixel, plane) pairs -- the batched form
  * of ComputeMultiViewCostVector (gmvs/lib/src/patch_match_kernel.cu:571-588,
  * gmvs/src/patch_match.cu:503-522). xy: device [n*2] int32, planes: device

@@ -255,8 +255,22 @@ def main() -> int:
     geom_per_launch = dom["geom"] / dom["launches"]
     flops = ncc_per_launch * FLOPS_PER_NCC + geom_per_launch * FLOPS_PER_GEOM
     ach = flops / (avg_ms * 1e-3) / 1e12
+    # HBM bytes per launch of that kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes over this workload (scaled by a copy of
+    # known size); counters cannot be read inside a timed run
+    traffic, hbm = None, None
+    tpath = os.path.join(HERE, "profiles", "pm_traffic_acmp_10v_1600x1200.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tk = json.load(f)["kernels"].get(dom_name)
+        if tk:
+            traffic = tk["traffic_bytes_per_launch"]
+            gbs = traffic / (avg_ms * 1e-3) / 1e9
+            hbm = {"achieved_GBps": round(gbs, 1), "peak_GBps": PEAK_HBM_GBS,
+                   "frac": round(gbs / PEAK_HBM_GBS, 4)}
     roof = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+            "hbm": hbm,
             "kernel": dom_name, "avg_ms": round(avg_ms, 4),
             "ncc_per_launch": int(ncc_per_launch), "geom_per_launch": int(geom_per_launch),
             "gather_GBps": round(ncc_per_launch * GATHER_BYTES_PER_NCC / (avg_ms * 1e-3) / 1e9, 1)}

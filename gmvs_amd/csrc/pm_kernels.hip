@@ -630,7 +630,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
             const float m[3] = {s_m[k][0][threadIdx.x], s_m[k][1][threadIdx.x], s_m[k][2][threadIdx.x]};
             float H[9];
             homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval<NT, L, PR>(hs, H, src, P.h, P.w);
+            const float cv = ncc_eval_mlp<NT, L, PR>(hs, H, src, P.h, P.w);
             if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
     }

@@ -169,6 +169,116 @@ def summarize_profile(prof_dir: str) -> str:
     return "\n".join(out)
 
 
+SCENES4 = [  # config 4: (width, height, planes, plane depth range)
+    (1024, 768, 16, (2.0, 6.0)),
+    (1600, 1200, 27, (2.5, 7.0)),
+    (2048, 1536, 38, (3.0, 8.0)),
+    (3200, 2400, 48, (2.0, 5.0)),
+]
+
+
+def run_scenes4(args) -> int:
+    """Config 4: 4 synthetic scenes of 12 views at 1024x768, 1600x1200,
+    2048x1536 and 3200x2400 (48 reference views, 11 sources each, plain
+    pass, 8 iterations), every view as reference. Timed: the plain path (FP32,
+    one view per call) and the whole 48-view batch replayed from one HIP graph
+    in FP32, FP16 and BF16, tile shape and blocks per CU from the autotuner."""
+    import numpy as np
+    import torch
+
+    from gmvs_amd import _native as N
+    from gmvs_amd import src as S
+    from gmvs_amd.autotune import autotune
+    from gmvs_amd.pipeline import cameras_from_synthetic
+    from gmvs_amd.synthetic import make_scene
+
+    dev = "cuda"
+    problems, ph = [], 0
+    for i, (w, h, npl, dr) in enumerate(SCENES4):
+        log(f"scene {i}: {w}x{h}, {npl} planes, 12 views")
+        sc = make_scene(w, h, 12, npl, seed=3407 + 40 + i, depth_range=dr,
+                        cam_depth_range=(dr[0] + 0.25, dr[1]), device=dev)
+        cams = cameras_from_synthetic(sc.cameras)
+        imgs = sc.images.to(dev, torch.float32).contiguous()
+        for ref in range(12):
+            ids = [ref] + list(sc.pairs[ref][:11])
+            prm = S.pm_params_for_problem(S.PatchMatchParams(), cams[ref], len(ids))
+            prm.max_iterations = 8
+            problems.append(dict(cams=[cams[j]._c for j in ids], images=[imgs[j] for j in ids],
+                                 params=prm, init_mode=N.INIT_RANDOM, geom=False, rng_view=ref,
+                                 planes=torch.zeros(h, w, 4, device=dev),
+                                 costs=torch.zeros(h, w, device=dev)))
+            ph += h * w * (1 + 14 * 8)
+        del sc
+    torch.cuda.synchronize()
+
+    results = {}
+    log("plain FP32, one view per call")
+    ws = S._Workspace()
+
+    def plain():
+        for p in problems:
+            kw = {k: v for k, v in p.items() if k not in ("cams", "images", "params")}
+            S.run_pm(p["cams"], p["images"], p["params"], workspace=ws, **kw)
+
+    plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plain()
+    torch.cuda.synchronize()
+    plain_rate = ph * args.steps / (time.perf_counter() - t0) / 1e6
+    results["fp32_per_view"] = {"value": round(plain_rate, 2)}
+    log(f"fp32_per_view: {plain_rate:.1f} Mpix-hyp/s")
+    del ws
+
+    runs = (("fp32_default", N.PREC_FP32, False), ("fp32", N.PREC_FP32, True),
+            ("fp16", N.PREC_FP16, True), ("bf16", N.PREC_BF16, True))
+    for name, prec, tune in runs:
+        batch = [dict(p, precision=prec) for p in problems]
+        tuned = autotune(batch) if tune else {"tile_cols": 8, "blocks_candidates": 0,
+                                              "blocks_refine": 0}
+        batch = [dict(p, tile_cols=tuned["tile_cols"]) for p in batch]
+        ws = S._Workspace()
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            S.run_pm_batch(batch, ws)  # eager run sizes the workspace
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            S.run_pm_batch(batch, ws)
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        rate = ph * args.steps / (time.perf_counter() - t0) / 1e6
+        results[f"{name}_batched_graph"] = {"value": round(rate, 2),
+                                            "speedup_vs_plain_fp32": round(rate / plain_rate, 3),
+                                            "autotune": tuned}
+        log(f"{name} batched graph: {rate:.1f} Mpix-hyp/s ({rate / plain_rate:.3f}x), tuned {tuned}")
+        del g, ws
+        torch.cuda.empty_cache()
+        N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_CANDIDATES, 0))
+        N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_REFINE, 0))
+    N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_CANDIDATES, 0))
+    N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_REFINE, 0))
+    best = results["bf16_batched_graph"]["value"]
+    line = {
+        "metric": "PatchMatch Mpix-hyp/s + fusion Mpts/s; batched BF16 vs plain FP32 at fixed err",
+        "value": best, "unit": "Mpix-hyp/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(ph / best / 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "scenes4_48v_uneven_plain_8it_graph", "views": len(problems),
+                   "sizes": [f"{w}x{h}" for w, h, _, _ in SCENES4], "sources": 11,
+                   "iterations": 8, "pixel_hypotheses_per_step": ph},
+        "variants": results,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +289,9 @@ def main() -> int:
     ap.add_argument("--variants", default="plain,fp16,bf16",
                     help="other runs of the same workload: plain (FP32, one view per call), "
                          "fp16 / bf16 (batched, 16-bit samples); comma list, empty for none")
+    ap.add_argument("--workload", default="acmp10", choices=("acmp10", "scenes4"),
+                    help="acmp10: the 10-view ACMP pipeline (default); scenes4: 4 scenes x 12 "
+                         "views of uneven sizes in one batched HIP graph")
     ap.add_argument("--summarize-profile", metavar="DIR", default=None,
                     help="print a summary of rocprofv3 CSV output in DIR and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -192,6 +305,8 @@ def main() -> int:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not run)"}))
         return 2
+    if args.workload == "scenes4":
+        return run_scenes4(args)
 
     import numpy as np
     import torch

@@ -1,0 +1,111 @@
+"""Autotuner of the PatchMatch launch shape.
+
+Two knobs change how the work is laid out on the GPU but never the results:
+
+* the tile shape of the 64-pixel work tiles, per problem (`tile_cols` of
+  gmvs_pm_args: 4, 8, 16 or 32 columns);
+* the blocks per CU of each persistent kernel (gmvs_set_blocks_per_cu: 0 =
+  as many as fit).
+
+`autotune(problems)` times short plain passes (random initialisation, one
+iteration) of the given batch on scratch state for each setting and keeps
+the fastest. Results are cached per batch shape for the process, so a
+pipeline tunes once.
+"""
+from __future__ import annotations
+
+from typing import Dict, Optional, Sequence, Tuple
+
+import torch
+
+from . import _native as N
+from . import src as S
+
+TILE_COLS = (4, 8, 16, 32)
+BLOCKS = (0, 3, 2)
+
+_cache: Dict[Tuple, dict] = {}
+
+
+def _key(problems: Sequence[dict]) -> Tuple:
+    return tuple((tuple(p["images"][0].shape), len(p["images"]), p.get("precision", N.PREC_FP32))
+                 for p in problems)
+
+
+def _trial(problems: Sequence[dict], tile_cols: int, iterations: int) -> list:
+    out = []
+    for p in problems:
+        cp = N.CParams.from_buffer_copy(p["params"])
+        cp.max_iterations = iterations
+        cp.planar_prior = False
+        cp.geom_consistency = False
+        out.append(dict(cams=p["cams"], images=p["images"], params=cp, init_mode=N.INIT_RANDOM,
+                        geom=False, rng_view=p.get("rng_view", 0),
+                        planes=torch.empty_like(p["planes"]), costs=torch.empty_like(p["costs"]),
+                        precision=p.get("precision", N.PREC_FP32), tile_cols=tile_cols))
+    return out
+
+
+def _time(problems: Sequence[dict], ws: S._Workspace, kernel: Optional[str] = None,
+          repeats: int = 2) -> float:
+    """Best of `repeats` timed runs after a warm run: the whole batch (HIP
+    events) or one kernel's accumulated time (the library's profiler)."""
+    S.run_pm_batch(problems, ws)  # warm (and sizes the workspace)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(repeats):
+        if kernel is None:
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            S.run_pm_batch(problems, ws)
+            b.record()
+            torch.cuda.synchronize()
+            best = min(best, a.elapsed_time(b))
+        else:
+            N.profile_reset()
+            N.profile_enable(True)
+            S.run_pm_batch(problems, ws)
+            torch.cuda.synchronize()
+            N.profile_enable(False)
+            best = min(best, N.profile_collect()[kernel]["ms"])
+    return best
+
+
+def _pick(times: Dict[int, float], default: int, margin: float) -> int:
+    """The fastest setting, unless it beats the default by less than
+    `margin` (relative): short timings are noisy, and the default is the one
+    measured on full workloads."""
+    best = min(times, key=times.get)
+    return best if times[best] < times[default] * (1.0 - margin) else default
+
+
+def autotune(problems: Sequence[dict], iterations: int = 2, tile_cols: Sequence[int] = TILE_COLS,
+             blocks: Sequence[int] = BLOCKS, apply: bool = True, margin: float = 0.03) -> dict:
+    """Best {'tile_cols', 'blocks_candidates', 'blocks_refine'} for a batch of
+    make_pm_args keyword dicts, keeping the defaults (8 x 8 tiles, full
+    occupancy) unless another setting is faster by more than `margin`; with
+    apply=True the blocks per CU are set process-wide. The caller's state
+    buffers are not touched."""
+    key = _key(problems)
+    if key not in _cache:
+        ws = S._Workspace()
+        tt = {tc: _time(_trial(problems, tc, iterations), ws) for tc in tile_cols}
+        best_tc = _pick(tt, 8 if 8 in tt else tile_cols[0], margin)
+        trial = _trial(problems, best_tc, iterations)
+        choice = {"tile_cols": best_tc}
+        for name, tune_id, kname in (("blocks_candidates", N.TUNE_CANDIDATES, "pm_candidates"),
+                                     ("blocks_refine", N.TUNE_REFINE, "pm_refine")):
+            times = {}
+            for b in blocks:
+                N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(tune_id, b))
+                times[b] = _time(trial, ws, kname)
+            choice[name] = _pick(times, 0 if 0 in times else blocks[0], margin)
+            N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(tune_id, 0))
+        _cache[key] = choice
+    choice = _cache[key]
+    if apply:
+        N.check("gmvs_set_blocks_per_cu",
+                N.lib.gmvs_set_blocks_per_cu(N.TUNE_CANDIDATES, choice["blocks_candidates"]))
+        N.check("gmvs_set_blocks_per_cu",
+                N.lib.gmvs_set_blocks_per_cu(N.TUNE_REFINE, choice["blocks_refine"]))
+    return dict(choice)

@@ -4,7 +4,8 @@ Every translation unit is compiled with hipcc for gfx950 only; packed FP32
 instructions are disabled (-packed-fp32-ops). FP32 division/sqrt are
 correctly rounded except in pm_kernels.hip, whose results are checked against
 the FP64 oracle within a tolerance. fusion.hip additionally disables FMA
-contraction so that its arithmetic matches a sequential CPU loop bit for bit.
+contraction so that its arithmetic matches a sequential CPU loop bit for bit;
+so does post.hip (point gather, text export, segment utilities).
 """
 from __future__ import annotations
 
@@ -42,6 +43,7 @@ SOURCES = {
     # costs are compared against the FP64 oracle within a tolerance
     "pm_kernels.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "fusion.hip": ["-ffp-contract=off"],
+    "post.hip": ["-ffp-contract=off"],
 }
 
 

@@ -117,11 +117,40 @@ EXPORTS = (
     "gmvs_fuse_view",
     "gmvs_prior_build",
     "gmvs_delaunay",
+    "gmvs_points_workspace_bytes",
+    "gmvs_points_gather",
+    "gmvs_format_workspace_bytes",
+    "gmvs_format_rows",
+    "gmvs_sliding_window_normal",
+    "gmvs_segments_workspace_bytes",
+    "gmvs_filter_by_var_map",
+    "gmvs_fusion_planar_mask",
     "gmvs_profile_enable",
     "gmvs_profile_collect",
     "gmvs_profile_reset",
     "gmvs_philox4x32_10",
 )
+
+
+class CPointsArgs(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("num_views", ctypes.c_int32),
+        ("depth_normals", ctypes.POINTER(ctypes.c_void_p)),
+        ("poses", ctypes.c_void_p),
+        ("fx", ctypes.c_double),
+        ("fy", ctypes.c_double),
+        ("cx", ctypes.c_double),
+        ("cy", ctypes.c_double),
+        ("cos_view_angle", ctypes.c_double),
+        ("filtered", ctypes.c_void_p),
+        ("points", ctypes.c_void_p),
+        ("points_filtered", ctypes.c_void_p),
+        ("capacity", ctypes.c_uint64),
+        ("capacity_filtered", ctypes.c_uint64),
+        ("counts", ctypes.c_void_p),
+    ]
 
 
 class CKernelStat(ctypes.Structure):
@@ -169,6 +198,19 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_prior_build.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,
                                      ctypes.POINTER(CParams), vp, vp, ctypes.POINTER(i32)]
     lib.gmvs_delaunay.argtypes = [vp, i32, vp, i32, ctypes.POINTER(i32)]
+    u64 = ctypes.c_uint64
+    lib.gmvs_points_workspace_bytes.argtypes = [i32, i32, i32]
+    lib.gmvs_points_workspace_bytes.restype = sz
+    lib.gmvs_points_gather.argtypes = [ctypes.POINTER(CPointsArgs), vp, sz, vp]
+    lib.gmvs_format_workspace_bytes.argtypes = [u64]
+    lib.gmvs_format_workspace_bytes.restype = sz
+    lib.gmvs_format_rows.argtypes = [vp, u64, i32, i32, vp, u64, vp, vp, sz, vp]
+    lib.gmvs_sliding_window_normal.argtypes = [vp, vp, i32, i32, i32, vp, vp]
+    lib.gmvs_segments_workspace_bytes.argtypes = [i32, i32, i32]
+    lib.gmvs_segments_workspace_bytes.restype = sz
+    lib.gmvs_filter_by_var_map.argtypes = [vp, vp, i32, i32, i32, f32, vp, vp, sz, vp]
+    lib.gmvs_fusion_planar_mask.argtypes = [vp, vp, vp, i32, i32, i32, i32, f32, f32, f32, vp, vp,
+                                            sz, vp]
     lib.gmvs_philox4x32_10.argtypes = [vp, vp, vp]
     lib.gmvs_philox4x32_10.restype = None
     lib.gmvs_profile_enable.argtypes = [i32]

@@ -32,6 +32,11 @@ enum KernelId {
     K_FUSE_CHECK,
     K_FUSE_SCATTER,
     K_FUSE_VIEWDESC,
+    K_POINTS,
+    K_FORMAT,
+    K_SWN,
+    K_SEG_SORT,
+    K_SEG_STATS,
     K_NUM
 };
 

@@ -7,7 +7,7 @@ pass, then (after ``triangulation`` + ``cuda_planar_prior_init``) the
 planar-prior pass. Images live in plain device buffers (no textures); the
 hypothesis state stays on the device between the two passes.
 ``fusion_planar_mask``, ``filter_by_var_map`` and ``sliding_window_normal``
-are mesh/segmentation utilities outside this hot path and raise.
+run on the GPU (gmvs_amd.post).
 """
 from __future__ import annotations
 
@@ -19,6 +19,7 @@ import torch
 
 from ... import _native as N
 from ... import io as gio
+from ... import post as _post
 from ...src import (Camera, PatchMatchParams, Problem, _Workspace, _device, load_camera,
                     load_problem, run_pm)
 
@@ -192,14 +193,8 @@ class PlanarInit:
         Image.fromarray(rgb).save(path)
 
 
-def fusion_planar_mask(*args, **kwargs):
-    raise NotImplementedError("fusion_planar_mask is a mesh post-processing utility outside "
-                              "the PatchMatch hot path")
-
-
-def filter_by_var_map(*args, **kwargs):
-    raise NotImplementedError("filter_by_var_map is outside the PatchMatch hot path")
-
-
-def sliding_window_normal(*args, **kwargs):
-    raise NotImplementedError("sliding_window_normal is outside the PatchMatch hot path")
+# The segment / normal utilities of the module (planar_initer.cu:471-684),
+# on the GPU; numpy in, numpy out, as the reference's cv::Mat bindings.
+fusion_planar_mask = _post.fusion_planar_mask
+filter_by_var_map = _post.filter_by_var_map
+sliding_window_normal = _post.sliding_window_normal

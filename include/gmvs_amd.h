@@ -210,6 +210,81 @@ int gmvs_prior_build(const gmvs_camera* ref_camera, const float* depth, const fl
 int gmvs_delaunay(const int32_t* xy, int32_t n, int32_t* tri_out, int32_t max_tri,
                   int32_t* num_tri);
 
+/* Point gather after fusion -- the "Gather mvs points" loop of
+ * gmvs/scripts/mvs_fusion_segmentaion_cpu.py:158-205. For each view, the
+ * pixels of its fused [H,W,4] depth_normal map with depth > 0 become rows
+ * [x y z nx ny nz] (float64): point = C + depth * R (x', y', 1) with
+ * x' = (col - cx) / fx, y' = (row - cy) / fy, (R | C) the camera-to-world pose
+ * (pose/NNNN.txt) and the normal widened from float32. A row passes the
+ * view-angle filter when -(R (x', y', 1)) . n > cos_view_angle (the ray is not
+ * normalised, as in the reference; cos(70 deg) there). Rows come in view
+ * order, then row-major pixel order (numpy's boolean-mask order).
+ * depth_normals: host array of num_views device pointers [H*W*4];
+ * poses: host [num_views*16] row-major 4x4; filtered: device
+ * [num_views*H*W*4] or NULL (depth_normal_filter: the map where the filter
+ * passes, else 0); points / points_filtered: device [capacity*6] /
+ * [capacity_filtered*6] or NULL (rows past the capacity are dropped);
+ * counts: device uint64 [2] receives the total and filtered row counts.
+ * Calling with no outputs counts only. */
+typedef struct gmvs_points_args {
+    int32_t height, width, num_views;
+    const float* const* depth_normals;
+    const double* poses;
+    double fx, fy, cx, cy;
+    double cos_view_angle;
+    float* filtered;
+    double* points;
+    double* points_filtered;
+    uint64_t capacity, capacity_filtered;
+    uint64_t* counts;
+} gmvs_points_args;
+size_t gmvs_points_workspace_bytes(int32_t height, int32_t width, int32_t num_views);
+int gmvs_points_gather(const gmvs_points_args* args, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
+/* Text export -- np.savetxt(path, rows, fmt="%.<decimals>f") as the
+ * reference writes mvs_depthnorm[_filter].xyz
+ * (mvs_fusion_segmentaion_cpu.py:207-216): values joined by ' ', rows ended
+ * by '\n', each value the exactly rounded decimal (ties to even) as Python's
+ * "%f" prints it, "-" on every negative value, nan / inf / -inf.
+ * rows: device [n_rows*ncol] float64; out: device [capacity] bytes or NULL
+ * (bytes past the capacity are dropped); out_len: device uint64 receiving
+ * the byte count, or UINT64_MAX if a value's magnitude times 10^decimals
+ * reaches 2^63 (unsupported). ncol <= 64, decimals <= 9. */
+size_t gmvs_format_workspace_bytes(uint64_t n_rows);
+int gmvs_format_rows(const double* rows, uint64_t n_rows, int32_t ncol, int32_t decimals, char* out,
+                     uint64_t capacity, uint64_t* out_len, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* sliding_window_normal (gmvs/lib/src/planar_initer.cu:471-535): device
+ * planar_region [H*W] int32 (1 = planar), normal [H*W*3] f32, out
+ * [H*W*3] f32: 0 on planar pixels, else the mean normal of the non-planar
+ * pixels of the window_size x window_size window, summed row by row and
+ * divided as cv::Vec3f /= int does. */
+int gmvs_sliding_window_normal(const int32_t* planar_region, const float* normal, int32_t height,
+                               int32_t width, int32_t window_size, float* out, void* stream);
+
+/* Per-superpixel utilities. segment ids: device [H*W] int32 in
+ * [0, num_segments) with num_segments = max id + 1 (ids outside count as
+ * segment 0); per-segment sums run over the segment's pixels in row-major
+ * order, as the reference's loops do. Not capturable in a HIP graph (the
+ * radix sort clears its counters with hipMemsetAsync). */
+size_t gmvs_segments_workspace_bytes(int32_t height, int32_t width, int32_t num_segments);
+/* filter_by_var_map (gmvs/lib/src/planar_initer.cu:634-684): out[p] = id
+ * if the mean of var_map over segment id (>= 1) is <= var_thresh, else 0. */
+int gmvs_filter_by_var_map(const float* var_map, const int32_t* segment_ids, int32_t height,
+                           int32_t width, int32_t num_segments, float var_thresh, int32_t* out,
+                           void* workspace, size_t workspace_bytes, void* stream);
+/* fusion_planar_mask (gmvs/lib/src/planar_initer.cu:537-632): planar_mask
+ * [H*W] int32 (0 = non-planar), filter_normals [H*W*3] f32; out [H*W] int32:
+ * 0 (small or empty segment), the segment id, 300 (object) or 301, 302, ...
+ * (textureless segments in id order). */
+int gmvs_fusion_planar_mask(const int32_t* seg_ids, const int32_t* planar_mask,
+                            const float* filter_normals, int32_t rows, int32_t cols,
+                            int32_t num_segments, int32_t thresh, float nonplanar_percent,
+                            float cos_sim_thresh, float match_ratio_thresh, int32_t* out,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
 /* Kernel timing. No counterpart in the reference (which only prints
  * wall-clock totals, gmvs/scripts/launch.py:125-126,176-177). When enabled,
  * every kernel launch of the library is bracketed by hipEvents on its

@@ -9,6 +9,9 @@ this module, and only as the checker (or the timed CPU baseline). The product
     cost_batch(...)    FP64 bilateral NCC of (pixel, plane) queries
     geom_batch(...)    FP64 geometric-consistency cost of (pixel, plane) queries
     fuse_view(...)     sequential FP32 fusion twin (bit-exact reference)
+    sliding_window_normal / filter_by_var_map / fusion_planar_mask(...)
+                       sequential twins of the post-fusion utilities
+    gather_points(...) numpy restatement of the post-fusion point gather
     philox(...)        the oracle's own Philox4x32-10 block
 """
 from __future__ import annotations
@@ -81,6 +84,13 @@ def _load():
                                      ctypes.POINTER(vp), vp, ctypes.c_int, vp, ctypes.c_int,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
     lib.oracle_fuse_view.restype = ctypes.c_int64
+    i32, f32 = ctypes.c_int, ctypes.c_float
+    lib.oracle_sliding_window_normal.argtypes = [vp, vp, i32, i32, i32, vp]
+    lib.oracle_sliding_window_normal.restype = None
+    lib.oracle_filter_by_var_map.argtypes = [vp, vp, i32, i32, f32, vp]
+    lib.oracle_filter_by_var_map.restype = None
+    lib.oracle_fusion_planar_mask.argtypes = [vp, vp, vp, i32, i32, i32, f32, f32, f32, vp]
+    lib.oracle_fusion_planar_mask.restype = None
     return lib
 
 
@@ -242,3 +252,69 @@ def fuse_view(cams: Sequence[OCamera], depths: Sequence[np.ndarray],
                                _ptr(sids), geom_consistent, reproj_px, rel_depth, cos_angle,
                                _ptr(od), _ptr(on))
     return od, on, int(n)
+
+
+def sliding_window_normal(planar: np.ndarray, normal: np.ndarray, window_size: int) -> np.ndarray:
+    """planar_initer.cu:471-535, sequential."""
+    pl = np.ascontiguousarray(planar, np.int32)
+    nm = _f32(normal)
+    h, w = pl.shape
+    out = np.zeros((h, w, 3), np.float32)
+    lib().oracle_sliding_window_normal(_ptr(pl), _ptr(nm), h, w, int(window_size), _ptr(out))
+    return out
+
+
+def filter_by_var_map(var_map: np.ndarray, seg: np.ndarray, var_thresh: float) -> np.ndarray:
+    """planar_initer.cu:634-684, sequential."""
+    v = _f32(var_map)
+    s = np.ascontiguousarray(seg, np.int32)
+    h, w = s.shape
+    out = np.zeros((h, w), np.int32)
+    lib().oracle_filter_by_var_map(_ptr(v), _ptr(s), h, w, float(var_thresh), _ptr(out))
+    return out
+
+
+def fusion_planar_mask(seg: np.ndarray, planar: np.ndarray, normals: np.ndarray, thresh: int,
+                       nonplanar_percent: float, cos_sim_thresh: float,
+                       match_ratio_thresh: float) -> np.ndarray:
+    """planar_initer.cu:537-632, sequential."""
+    s = np.ascontiguousarray(seg, np.int32)
+    pl = np.ascontiguousarray(planar, np.int32)
+    nm = _f32(normals)
+    rows, cols = s.shape
+    out = np.zeros((rows, cols), np.int32)
+    lib().oracle_fusion_planar_mask(_ptr(s), _ptr(pl), _ptr(nm), rows, cols, int(thresh),
+                                    float(nonplanar_percent), float(cos_sim_thresh),
+                                    float(match_ratio_thresh), _ptr(out))
+    return out
+
+
+def gather_points(depth_normals, poses, intrinsic, height: int, width: int):
+    """The gather loop of mvs_fusion_segmentaion_cpu.py:158-205 restated in
+    numpy (numpy >= 2 promotion: the float32 pixel grid against float64
+    intrinsics gives float64 rays). Returns (points [M,6], filtered points
+    [Mf,6], filtered maps [V,H,W,4] float32)."""
+    K = np.asarray(intrinsic, np.float64)
+    fx, fy, cx, cy = K[0, 0], K[1, 1], K[0, 2], K[1, 2]
+    yy, xx = np.meshgrid(np.arange(height, dtype=np.float32), np.arange(width, dtype=np.float32),
+                         indexing="ij")
+    xx = (xx - cx) / fx
+    yy = (yy - cy) / fy
+    dirs_ = np.stack((xx, yy, np.ones_like(xx)), axis=-1).reshape(-1, 3)
+    pts_all, pts_f, maps = [], [], []
+    for pose, dn in zip(poses, depth_normals):
+        E = np.asarray(pose, np.float64).reshape(4, 4)
+        origins = np.tile(E[:3, 3], (height * width, 1))
+        dirs = dirs_ @ E[:3, :3].T
+        depth = dn[..., :1].reshape(-1, 1)
+        normal = dn[..., 1:].reshape(-1, 3)
+        m = (depth > 0).squeeze()
+        points = origins[m] + depth[m] * dirs[m]
+        vm = ((-dirs[m]) * normal[m]).sum(-1) > np.cos(70 / 180 * np.pi)
+        keep = np.zeros(height * width, bool)
+        keep[np.flatnonzero(m)[vm]] = True
+        maps.append(np.where(keep.reshape(height, width, 1), dn, 0).astype(np.float32))
+        pn = np.concatenate((points, normal[m]), axis=1)
+        pts_all.append(pn)
+        pts_f.append(pn[vm])
+    return np.concatenate(pts_all), np.concatenate(pts_f), np.stack(maps)

@@ -1,0 +1,54 @@
+"""On-disk formats of the launch path, pinned by hand-written fixtures in the
+layout colmap2mvs.py:531-558 writes (tests/golden/cam_0003.txt,
+pair_small.txt): camera tokens 1-16 extrinsic, 18-26 K, 27 depth_min,
+30 depth_max (consistency.py:13-24); pair entries with a negative score are
+dropped (launch.py:22-50)."""
+import os
+
+import numpy as np
+import pytest
+
+from gmvs_amd import io as gio
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_read_camera_fixture():
+    K, R, t, (dmin, dmax) = gio.read_camera(os.path.join(GOLDEN, "cam_0003.txt"))
+    assert K == [300.0, 0.0, 160.0, 0.0, 301.5, 120.25, 0.0, 0.0, 1.0]
+    assert np.allclose(np.array(R).reshape(3, 3) @ np.array(R).reshape(3, 3).T, np.eye(3))
+    assert R[2] == pytest.approx(-0.08715574274765817)
+    assert t == pytest.approx([0.3486229709906327, -0.05, 4.0152345])
+    assert (dmin, dmax) == pytest.approx((2.5, 5.833333))
+
+
+def test_camera_round_trip(tmp_path):
+    K = np.array([[1500.0, 0, 800.5], [0, 1499.0, 600.25], [0, 0, 1]])
+    a = np.radians(17.0)
+    R = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    t = np.array([0.1, -0.2, 4.5])
+    p = tmp_path / "0007_cam.txt"
+    gio.write_camera(str(p), K, R, t, 1.5, 7.25)
+    K2, R2, t2, rng = gio.read_camera(str(p))
+    assert np.allclose(np.array(K2).reshape(3, 3), K)
+    assert np.allclose(np.array(R2).reshape(3, 3), R)
+    assert np.allclose(t2, t)
+    assert rng == pytest.approx([1.5, 7.25], abs=1e-6)
+
+
+def test_read_pair_fixture_drops_negative_scores():
+    pairs = gio.read_pair(os.path.join(GOLDEN, "pair_small.txt"))
+    assert pairs == [(0, [1]), (1, [0, 2]), (2, [1, 0])]
+
+
+def test_pair_round_trip(tmp_path):
+    p = tmp_path / "pair.txt"
+    gio.write_pair(str(p), [(0, [(2, 900), (1, 50)]), (1, [(0, 7)]), (2, [(0, 900)])])
+    assert gio.read_pair(str(p)) == [(0, [2, 1]), (1, [0]), (2, [0])]
+
+
+def test_gray_round_trip(tmp_path):
+    img = np.random.default_rng(0).integers(0, 256, (37, 53)).astype(np.float32)
+    p = tmp_path / "0000.png"
+    gio.write_gray(str(p), img)
+    assert np.array_equal(gio.read_gray(str(p)), img)

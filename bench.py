@@ -107,6 +107,78 @@ def fusion_workload(device: str):
     return sc, d.contiguous(), sc.normal.contiguous(), pairs
 
 
+def post_fusion(fsc, od, on, cpu_sample: bool):
+    """The stage after fusion (mvs_fusion_segmentaion_cpu.py:158-216) on the
+    fused maps: depth_normal maps as consistency.py:193-210 builds them, the
+    GPU point gather with its view-angle filter, and the %.4f text of both
+    point sets (formatting plus the copy of the text to the host). With
+    cpu_sample, the numpy gather and np.savetxt -- the reference's own code
+    path -- on a bounded sample: 2 views and the first 200,000 rows."""
+    import io as _io
+
+    import numpy as np
+    import torch
+
+    from gmvs_amd import post
+
+    nv, h, w = od.shape
+    nrm = on.clone()
+    nn = torch.linalg.norm(nrm, dim=-1, keepdim=True)
+    nrm = nrm / torch.where(nn > 0.5, nn, torch.ones_like(nn))
+    maps = torch.cat([od[..., None], nrm], dim=-1).contiguous()
+    poses = []
+    for c in fsc.cameras:
+        E = np.eye(4)
+        E[:3, :3] = c.R.T
+        E[:3, 3] = -c.R.T @ c.t
+        poses.append(E)
+    K = fsc.cameras[0].K
+    post.gather_points([maps[0]], poses[:1], K)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pts, pts_f, filt = post.gather_points(list(maps), poses, K)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    from gmvs_amd import _native as N
+
+    post.format_rows_device(pts_f[:1000])  # warm
+    torch.cuda.synchronize()
+    N.profile_reset()
+    N.profile_enable(True)
+    t0 = time.perf_counter()
+    texts = [post.format_rows_device(pts), post.format_rows_device(pts_f)]
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    N.profile_enable(False)
+    kms = N.profile_collect().get("format_rows", {}).get("ms", float("nan"))
+    t0 = time.perf_counter()
+    txt = sum(len(memoryview(t.cpu().numpy())) for t in texts)
+    th = time.perf_counter() - t0
+    del texts
+    nrows = int(pts.shape[0] + pts_f.shape[0])
+    res = {"input_points": nv * h * w, "points": int(pts.shape[0]), "filtered_points": int(pts_f.shape[0]),
+           "gather_ms": round(tg * 1e3, 2), "gather_mpts_per_s": round(nv * h * w / tg / 1e6, 1),
+           "text_bytes": txt, "format_ms": round(tf * 1e3, 2),
+           "format_mrows_per_s": round(nrows / tf / 1e6, 2), "format_kernels_ms": round(kms, 2),
+           "text_to_host_ms": round(th * 1e3, 2)}
+    del filt
+    if cpu_sample:
+        from oracle import oracle as O
+
+        m2 = maps[:2].cpu().numpy()
+        t0 = time.perf_counter()
+        O.gather_points(list(m2), poses[:2], K, h, w)
+        cg = time.perf_counter() - t0
+        rows = pts[:200000].cpu().numpy()
+        t0 = time.perf_counter()
+        np.savetxt(_io.BytesIO(), rows, fmt="%.4f")
+        cf = time.perf_counter() - t0
+        res["cpu_numpy"] = {"gather_mpts_per_s": round(2 * h * w / cg / 1e6, 2),
+                            "savetxt_mrows_per_s": round(len(rows) / cf / 1e6, 3), "cores": 1,
+                            "sample": "2 views gathered, 200,000 rows formatted"}
+    return res
+
+
 def _short_kernel(name: str) -> str:
     name = name.strip().strip('"')
     if name.startswith("void "):
@@ -448,6 +520,9 @@ def main() -> int:
         fusion = {"mpts_per_s": round(pts / ft / 1e6, 1), "input_points": pts,
                   "fused_points": int((od > 0).sum().item()), "ms": round(ft * 1e3, 2)}
         log(f"fusion: {fusion}")
+        fusion["post"] = post_fusion(fsc, od, on, not args.no_cpu_baseline)
+        log(f"post-fusion gather and text: {fusion['post']}")
+        del od, on, mk
 
     cpu = None
     if not args.no_cpu_baseline:

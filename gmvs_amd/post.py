@@ -91,9 +91,9 @@ def gather_points(depth_normals: Sequence[ArrayLike], poses: Sequence[np.ndarray
     return pts, pts_f, filtered
 
 
-def format_rows(rows: ArrayLike, decimals: int = 4) -> bytes:
-    """The bytes np.savetxt(f, rows, fmt=f"%.{decimals}f") writes for a 2-D
-    float64 array, formatted on the GPU."""
+def format_rows_device(rows: ArrayLike, decimals: int = 4) -> torch.Tensor:
+    """The text np.savetxt(f, rows, fmt=f"%.{decimals}f") writes for a 1-D or
+    2-D float64 array, formatted on the GPU: a device uint8 tensor."""
     dev = _device()
     r = _to_dev(rows, torch.float64, dev)
     if r.dim() == 1:
@@ -102,7 +102,7 @@ def format_rows(rows: ArrayLike, decimals: int = 4) -> bytes:
         raise ValueError("format_rows: rows must be 1-D or 2-D")
     n, ncol = int(r.shape[0]), int(r.shape[1])
     if n == 0:
-        return b""
+        return torch.empty(0, dtype=torch.uint8, device=dev)
     out_len = torch.zeros(1, dtype=torch.int64, device=dev)
     ws = _ws(N.lib.gmvs_format_workspace_bytes(n), dev)
     st = _stream(dev)
@@ -116,16 +116,21 @@ def format_rows(rows: ArrayLike, decimals: int = 4) -> bytes:
     N.check("gmvs_format_rows", N.lib.gmvs_format_rows(r.data_ptr(), n, ncol, decimals, out.data_ptr(),
                                                        total, out_len.data_ptr(), ws.data_ptr(),
                                                        ws.numel(), st))
-    return out.cpu().numpy().tobytes()
+    return out
+
+
+def format_rows(rows: ArrayLike, decimals: int = 4) -> bytes:
+    """format_rows_device, copied to the host as bytes."""
+    return format_rows_device(rows, decimals).cpu().numpy().tobytes()
 
 
 def savetxt(path: str, rows: ArrayLike, decimals: int = 4) -> int:
     """np.savetxt(path, rows, fmt=f"%.{decimals}f") with the formatting on the
     GPU; returns the bytes written."""
-    data = format_rows(rows, decimals)
+    text = format_rows_device(rows, decimals).cpu().numpy()
     with open(path, "wb") as f:
-        f.write(data)
-    return len(data)
+        f.write(memoryview(text))
+    return int(text.size)
 
 
 def _like_input(x: torch.Tensor, ref: ArrayLike):

@@ -2,6 +2,8 @@
 
 The library is built in-tree by ``gmvs_amd._build.build()``; importing this
 module loads it and raises if it is missing -- there is no fallback path.
+``GMVS_AMD_LIB`` names another build of the same library to load instead
+(e.g. one compiled with different launch bounds).
 """
 from __future__ import annotations
 
@@ -10,7 +12,7 @@ import os
 from typing import Optional, Sequence
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG_DIR, "libgmvs_amd.so")
+LIB_PATH = os.environ.get("GMVS_AMD_LIB") or os.path.join(_PKG_DIR, "libgmvs_amd.so")
 
 MAX_NGB = 32
 OK = 0

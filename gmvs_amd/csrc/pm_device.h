@@ -294,10 +294,12 @@ __device__ inline float geom_cost(const RefCam& rc, const ViewDesc& vd, float4 p
 template <int L>
 __device__ inline float group_sum(float v) {
     static_assert(L == 1 || L == 2 || L == 4, "lane group of 1, 2 or 4");
+    // update_dpp with old = 0 and bound_ctrl folds into one v_add_f32_dpp
+    // (every quad_perm source lane exists, so neither ever applies)
     if constexpr (L >= 2)  // quad_perm [1,0,3,2]
-        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
     if constexpr (L >= 4)  // quad_perm [2,3,0,1]
-        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
     return v;
 }
 
@@ -316,6 +318,23 @@ struct TapBlock {
         if constexpr (L == 4) return (sub & 1) * NC + c;
         else return c;
     }
+    // The extreme tap rows/columns of the patch that this lane owns: for
+    // L = 4 one row (row 0 or row NT-1) and one column, so the group's four
+    // lanes hold the four corners of the tap rectangle; for L = 2 one row
+    // and both columns; for L = 1 all four corners.
+    // (patch rows/columns 0 or NT-1, computed, not indexed: a lane-dependent
+    // index into the tap arrays would move them to scratch memory)
+    static constexpr int NKX = (L == 1) ? 2 : 1;
+    static constexpr int NKY = (L == 4) ? 1 : 2;
+    __device__ static int kx_row(int sub, int i) {
+        if constexpr (L == 1) return i == 0 ? 0 : NT - 1;
+        else if constexpr (L == 2) return sub == 0 ? 0 : NT - 1;
+        else return (sub >> 1) == 0 ? 0 : NT - 1;
+    }
+    __device__ static int ky_col(int sub, int i) {
+        if constexpr (L == 4) return (sub & 1) == 0 ? 0 : NT - 1;
+        else return i == 0 ? 0 : NT - 1;
+    }
 };
 
 template <int NT, int L>
@@ -325,9 +344,38 @@ struct Hoist {
     float wr[B::NR * B::NC];
     float tx[B::NR];  // clamped tap columns of the lane's rows (src 400-405)
     float ty[B::NC];  // clamped tap rows of the lane's columns
+    float kx[B::NKX];  // the lane's corners of the group's tap rectangle
+    float ky[B::NKY];
     float inv_w, mean_r, var_r, c;
     float px, py;
 };
+
+// Fills the tap coordinates of a lane (tx, ty, kx, ky).
+template <int NT, int L>
+__device__ inline void hoist_taps(Hoist<NT, L>& hs, int px, int py, int h, int w, int radius,
+                                  int inc, int sub) {
+    using B = TapBlock<NT, L>;
+#pragma unroll
+    for (int r = 0; r < B::NR; ++r)
+        hs.tx[r] = (float)min(max(px - radius + B::row(sub, r) * inc, 0), w - 1);
+#pragma unroll
+    for (int q = 0; q < B::NC; ++q)
+        hs.ty[q] = (float)min(max(py - radius + B::col(sub, q) * inc, 0), h - 1);
+    // (the empty asm keeps the compiler from re-deriving the corners as a
+    // lane-dependent pick among tx/ty, which it lowers through scratch memory)
+#pragma unroll
+    for (int i = 0; i < B::NKX; ++i) {
+        int k = B::kx_row(sub, i);
+        asm("" : "+v"(k));
+        hs.kx[i] = (float)min(max(px - radius + k * inc, 0), w - 1);
+    }
+#pragma unroll
+    for (int i = 0; i < B::NKY; ++i) {
+        int k = B::ky_col(sub, i);
+        asm("" : "+v"(k));
+        hs.ky[i] = (float)min(max(py - radius + k * inc, 0), h - 1);
+    }
+}
 
 template <int NT, int L>
 __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref, int px, int py,
@@ -340,12 +388,7 @@ __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref
     hs.c = c;
     const float ss = 2.0f * sigma_spatial * sigma_spatial;
     const float sc = 2.0f * sigma_color * sigma_color;
-#pragma unroll
-    for (int r = 0; r < B::NR; ++r)
-        hs.tx[r] = (float)min(max(px - radius + B::row(sub, r) * inc, 0), w - 1);
-#pragma unroll
-    for (int q = 0; q < B::NC; ++q)
-        hs.ty[q] = (float)min(max(py - radius + B::col(sub, q) * inc, 0), h - 1);
+    hoist_taps<NT, L>(hs, px, py, h, w, radius, inc, sub);
     float W = 0.f, R1 = 0.f, R2 = 0.f;
 #pragma unroll
     for (int r = 0; r < B::NR; ++r) {
@@ -397,12 +440,15 @@ constexpr int kBufWord3 = 0x00020000;
 // Storage and arithmetic of the source samples (GMVS_PREC_*):
 //   PREC_F32  -- cell (v00, v01-v00, v10-v00, v11-v10-v01+v00) in FP32,
 //                16 bytes; bilinear by three FP32 FMAs (the plain path);
-//   PREC_F16 / PREC_BF16 -- the same cell in FP16 / BF16, 8 bytes, and the
-//                bilinear sample s - c as two 16-bit dot products with FP32
-//                accumulation: dot((v00, v01-v00), (1, fx)) +
-//                dot((v10-v00, d11), (fy, fx*fy)) - c. Integer intensities
-//                up to 2048 (FP16) / 256 (BF16) and their differences are
-//                stored exactly; the weights carry the 16-bit rounding.
+//   PREC_F16  -- the same cell in FP16, 8 bytes; the bilinear sample in FP32
+//                arithmetic with the FP16 operands widened inside the FMA
+//                (v_fma_mix_f32). Integer intensities up to 2048 and their
+//                differences are stored exactly, so 8-bit images give the
+//                FP32 path's results bit for bit with half its gather bytes;
+//   PREC_BF16 -- the cell in BF16, 8 bytes, and the bilinear sample s - c as
+//                two BF16 dot products with FP32 accumulation:
+//                dot((v00, v01-v00), (1, fx)) + dot((v10-v00, d11), (fy, fx*fy))
+//                - c. The weights carry BF16 rounding (8-bit mantissa).
 // The NCC moments are accumulated in FP32 about the centre intensity in
 // every mode.
 enum : int { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
@@ -435,10 +481,10 @@ struct Cell<PREC_F16> {
         // (elements copied out first: clang's __builtin_bit_cast of a vector
         // element reads the vector's first element)
         const int q0 = q.x, q1 = q.y;
-        const f16x2 a = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(1.0f, fx));
-        const f16x2 b = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(fy, fx * fy));
-        const float lo = __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, q1), b, -c, false);
-        return __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, q0), a, lo, false);
+        const f16x2 a = __builtin_bit_cast(f16x2, q0), b = __builtin_bit_cast(f16x2, q1);
+        const float top = fmaf(fx, (float)a.y, (float)a.x);
+        const float dyv = fmaf(fx, (float)b.y, (float)b.x);
+        return fmaf(fy, dyv, top) - c;
     }
 };
 
@@ -476,50 +522,151 @@ __device__ inline QuadSrc quad_src(const void* cells, uint32_t count, int base) 
     return q;
 }
 
-// One bilateral NCC of the hoisted pixel against one source view. A tap is
-// one gather of a packed cell and its bilinear sample. Each lane issues the
-// gathers of a whole patch row before consuming them. All lanes of a group
-// return the same cost.
-template <int NT, int L, int PR>
-__device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
-                                 int h, int w) {
+// Source cell grid of one view: the image's h x w cells, padded with kPad
+// columns on the right and kPad rows at the bottom, row pitch wp = w + kPad.
+// A pad cell is a copy of the nearest edge cell; the edge cells' x (y)
+// differences are zero (the reference clamps the upper neighbour index), so
+// a tap beyond the right (bottom) edge samples the same value from a pad cell
+// at its own floor index as from the edge cell at the clamped index.
+constexpr int kPad = 32;
+
+struct SrcGrid {
+    int h, w;    // image size (bounds of the centre test and of the clamps)
+    int wp;      // cell row pitch, w + kPad
+    int fast;    // nonzero: clamp-free taps allowed
+};
+
+// Floor to int in one instruction (v_cvt_flr_i32_f32).
+__device__ inline int cvt_flr(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Clamp-free test of one lane group's patch: the lanes map the corners of
+// the tap rectangle (each lane its own, TapBlock::kx_row/ky_col); when z has
+// the centre's sign at every corner, z keeps that sign on the rectangle and
+// the projective image of every tap lies in the corners' convex hull. With
+// the corners inside [m, w + kPad - m] x [m, h + kPad - m] (m = 1/16 px,
+// far above the FP32 rounding of a tap position) every tap's floor cell is
+// inside the padded grid and none is left of or above the image, so the
+// taps need no clamping: the padded grid gives them the reference's clamped
+// values. Otherwise (and for NaN) false.
+template <int NT, int L>
+__device__ inline bool corners_in_grid(const Hoist<NT, L>& hs, const float H[9], float zc,
+                                       const SrcGrid& g) {
     using B = TapBlock<NT, L>;
-    const float rcz = __builtin_amdgcn_rcpf(fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8])));
+    constexpr float m = 0.0625f;
+    const float xmax = (float)(g.w + kPad) - m, ymax = (float)(g.h + kPad) - m;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < B::NKX; ++i) {
+        const float au = fmaf(H[0], hs.kx[i], H[2]);
+        const float av = fmaf(H[3], hs.kx[i], H[5]);
+        const float az = fmaf(H[6], hs.kx[i], H[8]);
+#pragma unroll
+        for (int j = 0; j < B::NKY; ++j) {
+            const float z = fmaf(H[7], hs.ky[j], az);
+            const float rz = __builtin_amdgcn_rcpf(z);
+            const float x = fmaf(H[1], hs.ky[j], au) * rz;
+            const float y = fmaf(H[4], hs.ky[j], av) * rz;
+            ok = ok && (z * zc > 0.0f) && (x >= m) && (x <= xmax) && (y >= m) && (y <= ymax);
+        }
+    }
+    return ok;
+}
+
+// One bilateral NCC of the hoisted pixel against one source view. A tap is
+// one gather of a packed cell and its bilinear sample. ALLFIRST: each lane
+// issues all its gathers before consuming the first (more loads in flight
+// per wave, for scattered footprints); else a patch row at a time. All
+// lanes of a group return the same cost.
+//
+// Tap index and fraction, two forms with identical results:
+//  - clamped (the reference's rule, src 16-31): index clamp(floor(x), 0,
+//    w-1), fraction x - index (negative left of the image: linear
+//    extrapolation; beyond the right edge the edge cell's zero differences
+//    repeat the edge value);
+//  - clamp-free, taken when every lane group of the wave passes
+//    corners_in_grid: floor(x) as the index into the padded grid and
+//    fract(x) = x - floor(x) as the fraction. Inside the image both forms
+//    are the same numbers; on the pad the value only depends on (v00, d10)
+//    or (v00, d01) of the edge cell, whose other terms are multiplied by
+//    exact zeros.
+template <int NT, int L, int PR, bool ALLFIRST>
+__device__ inline float ncc_eval_t(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+                                   const SrcGrid& g) {
+    using B = TapBlock<NT, L>;
+    constexpr int NG = ALLFIRST ? B::NR : 1;  // rows whose gathers are in flight together
+    const float zc = fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8]));
+    const float rcz = __builtin_amdgcn_rcpf(zc);
     const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
     const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
-    if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
-    const float wm1 = (float)(w - 1), hm1 = (float)(h - 1), wf = (float)w;
+    if (cx >= (float)g.w || cx < 0.0f || cy >= (float)g.h || cy < 0.0f) return 2.0f;
+    // wave-uniform choice (the ballot covers the lanes still active here)
+    const bool fast =
+        g.fast && __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L>(hs, H, zc, g)) == 0;
+    const float wm1 = (float)(g.w - 1), hm1 = (float)(g.h - 1), wpf = (float)g.wp;
     float S1 = 0.f, S2 = 0.f, X = 0.f;
 #pragma unroll
-    for (int r = 0; r < B::NR; ++r) {
-        const float au = fmaf(H[0], hs.tx[r], H[2]);
-        const float av = fmaf(H[3], hs.tx[r], H[5]);
-        const float az = fmaf(H[6], hs.tx[r], H[8]);
-        Cell<PR> q[B::NC];
-        float fx[B::NC], fy[B::NC];
+    for (int r0 = 0; r0 < B::NR; r0 += NG) {
+        Cell<PR> q[NG * B::NC];
+        float fx[NG * B::NC], fy[NG * B::NC];
+        if (fast) {
 #pragma unroll
-        for (int j = 0; j < B::NC; ++j) {
-            const float u = fmaf(H[1], hs.ty[j], au);
-            const float v = fmaf(H[4], hs.ty[j], av);
-            const float z = fmaf(H[7], hs.ty[j], az);
-            const float rz = __builtin_amdgcn_rcpf(z);
-            const float x = u * rz, y = v * rz;
-            // clamp(trunc(c), 0, n-1) of the reference equals
-            // clamp(floor(c), 0, n-1); any NaN/inf lands inside [0, n-1]
-            const float xf = __builtin_amdgcn_fmed3f(floorf(x), 0.f, wm1);
-            const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
-            fx[j] = x - xf;
-            fy[j] = y - yf;
-            q[j].load(src, (int)(uint32_t)fmaf(yf, wf, xf));
+            for (int rr = 0; rr < NG; ++rr) {
+                const int r = r0 + rr;
+                const float au = fmaf(H[0], hs.tx[r], H[2]);
+                const float av = fmaf(H[3], hs.tx[r], H[5]);
+                const float az = fmaf(H[6], hs.tx[r], H[8]);
+#pragma unroll
+                for (int j = 0; j < B::NC; ++j) {
+                    const float rz = __builtin_amdgcn_rcpf(fmaf(H[7], hs.ty[j], az));
+                    const float x = fmaf(H[1], hs.ty[j], au) * rz;
+                    const float y = fmaf(H[4], hs.ty[j], av) * rz;
+                    const int t = rr * B::NC + j;
+                    fx[t] = __builtin_amdgcn_fractf(x);
+                    fy[t] = __builtin_amdgcn_fractf(y);
+                    q[t].load(src, (int)(__umul24((unsigned)cvt_flr(y), (unsigned)g.wp) + (unsigned)cvt_flr(x)));
+                }
+            }
+        } else {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int rr = 0; rr < NG; ++rr) {
+                const int r = r0 + rr;
+                const float au = fmaf(H[0], hs.tx[r], H[2]);
+                const float av = fmaf(H[3], hs.tx[r], H[5]);
+                const float az = fmaf(H[6], hs.tx[r], H[8]);
+#pragma unroll
+                for (int j = 0; j < B::NC; ++j) {
+                    const float rz = __builtin_amdgcn_rcpf(fmaf(H[7], hs.ty[j], az));
+                    const float x = fmaf(H[1], hs.ty[j], au) * rz;
+                    const float y = fmaf(H[4], hs.ty[j], av) * rz;
+                    // clamp(trunc(c), 0, n-1) of the reference equals
+                    // clamp(floor(c), 0, n-1); any NaN/inf lands inside
+                    // [0, n-1]. No contraction: the fraction is formed from
+                    // the rounded x, as fract() is in the clamp-free form.
+                    const float xf = __builtin_amdgcn_fmed3f(floorf(x), 0.f, wm1);
+                    const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
+                    const int t = rr * B::NC + j;
+                    fx[t] = x - xf;
+                    fy[t] = y - yf;
+                    q[t].load(src, (int)(uint32_t)fmaf(yf, wpf, xf));
+                }
+            }
         }
 #pragma unroll
-        for (int j = 0; j < B::NC; ++j) {
-            const float s = q[j].sample(fx[j], fy[j], hs.c);
-            const float ws = hs.wt[r * B::NC + j] * s;
-            S1 += ws;
-            S2 = fmaf(ws, s, S2);
-            X = fmaf(hs.wr[r * B::NC + j], s, X);
-        }
+        for (int rr = 0; rr < NG; ++rr)
+#pragma unroll
+            for (int j = 0; j < B::NC; ++j) {
+                const int t = rr * B::NC + j;
+                const float s = q[t].sample(fx[t], fy[t], hs.c);
+                const float ws = hs.wt[(r0 + rr) * B::NC + j] * s;
+                S1 += ws;
+                S2 = fmaf(ws, s, S2);
+                X = fmaf(hs.wr[(r0 + rr) * B::NC + j], s, X);
+            }
     }
     S1 = group_sum<L>(S1);
     S2 = group_sum<L>(S2);
@@ -531,56 +678,16 @@ __device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const
     return fmaxf(0.0f, fminf(2.0f, 1.0f - cov * __builtin_amdgcn_rsqf(hs.var_r * var_s)));
 }
 
-// ncc_eval with every gather of the lane issued before the first one is
-// consumed: more loads in flight per wave, for hypotheses whose footprints
-// are scattered (random depths) and mostly miss the L1.
+template <int NT, int L, int PR>
+__device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+                                 const SrcGrid& g) {
+    return ncc_eval_t<NT, L, PR, false>(hs, H, src, g);
+}
+
 template <int NT, int L, int PR>
 __device__ inline float ncc_eval_mlp(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
-                                     int h, int w) {
-    using B = TapBlock<NT, L>;
-    const float rcz = __builtin_amdgcn_rcpf(fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8])));
-    const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
-    const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
-    if (cx >= (float)w || cx < 0.0f || cy >= (float)h || cy < 0.0f) return 2.0f;
-    const float wm1 = (float)(w - 1), hm1 = (float)(h - 1), wf = (float)w;
-    Cell<PR> q[B::NR * B::NC];
-    float fx[B::NR * B::NC], fy[B::NR * B::NC];
-#pragma unroll
-    for (int r = 0; r < B::NR; ++r) {
-        const float au = fmaf(H[0], hs.tx[r], H[2]);
-        const float av = fmaf(H[3], hs.tx[r], H[5]);
-        const float az = fmaf(H[6], hs.tx[r], H[8]);
-#pragma unroll
-        for (int j = 0; j < B::NC; ++j) {
-            const float u = fmaf(H[1], hs.ty[j], au);
-            const float v = fmaf(H[4], hs.ty[j], av);
-            const float z = fmaf(H[7], hs.ty[j], az);
-            const float rz = __builtin_amdgcn_rcpf(z);
-            const float x = u * rz, y = v * rz;
-            const float xf = __builtin_amdgcn_fmed3f(floorf(x), 0.f, wm1);
-            const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
-            fx[r * B::NC + j] = x - xf;
-            fy[r * B::NC + j] = y - yf;
-            q[r * B::NC + j].load(src, (int)(uint32_t)fmaf(yf, wf, xf));
-        }
-    }
-    float S1 = 0.f, S2 = 0.f, X = 0.f;
-#pragma unroll
-    for (int t = 0; t < B::NR * B::NC; ++t) {
-        const float s = q[t].sample(fx[t], fy[t], hs.c);
-        const float ws = hs.wt[t] * s;
-        S1 += ws;
-        S2 = fmaf(ws, s, S2);
-        X = fmaf(hs.wr[t], s, X);
-    }
-    S1 = group_sum<L>(S1);
-    S2 = group_sum<L>(S2);
-    X = group_sum<L>(X);
-    const float ms = S1 * hs.inv_w;
-    const float var_s = S2 * hs.inv_w - ms * ms;
-    if (hs.var_r < 1e-5f || var_s < 1e-5f) return 2.0f;
-    const float cov = X * hs.inv_w - hs.mean_r * ms;
-    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov * __builtin_amdgcn_rsqf(hs.var_r * var_s)));
+                                     const SrcGrid& g) {
+    return ncc_eval_t<NT, L, PR, true>(hs, H, src, g);
 }
 
 }  // namespace gmvs

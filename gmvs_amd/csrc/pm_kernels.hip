@@ -57,12 +57,14 @@ struct PMConst {
     RefCam rc;
     const float* ref;             // reference intensities [H*W]
     const ViewDesc* views;        // [nv]
-    const void* cells;            // packed cells of all views, [nv][H*W]
+    const void* cells;            // packed cells of all views, [nv][(H+kPad)*(W+kPad)]
     const float* init_depth;      // INIT_DEPTH_NORMAL
     const float* init_normal;     // INIT_DEPTH_NORMAL, world frame
     const float4* prior_planes;   // planar prior
     const uint32_t* plane_masks;  // planar prior
     int nv, h, w, wh;             // wh = active columns per row
+    int wp, vcells;               // cell row pitch w + kPad; cells per view (h + kPad) * wp
+    int fast_taps;                // clamp-free tap path allowed (SrcGrid::fast)
     int radius, inc;
     float sigma_s, sigma_c;
     float dmin, dmax;
@@ -70,6 +72,7 @@ struct PMConst {
     uint32_t seed, view;
     int init_mode, geom, prior;
     TileShape tile;
+    __device__ SrcGrid grid() const { return SrcGrid{h, w, wp, fast_taps}; }
 };
 
 // Adds per-lane work counts (NCCs, geometric terms) to a kernel's counters:
@@ -239,13 +242,16 @@ __global__ void pm_setup_view(gmvs_camera r, gmvs_camera s, const void* cells,
 // v11-v10-v01+v00), with the right/bottom neighbours clamped to the last
 // column/row as the reference's software bilinear does (src 20-21). The
 // differences are formed in FP32, then stored in the cell precision
-// (round to nearest even).
+// (round to nearest even). The grid is (h + kPad) x (w + kPad), row pitch
+// w + kPad; pad cells copy the nearest edge cell (see SrcGrid).
 template <int PR>
 __global__ void pm_pack_quads(const float* __restrict__ img, void* __restrict__ cells, int h,
                               int w) {
+    const int wp = w + kPad;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= h * w) return;
-    const int y = i / w, x = i - y * w;
+    if (i >= (h + kPad) * wp) return;
+    const int yp = i / wp, xp = i - yp * wp;
+    const int y = min(yp, h - 1), x = min(xp, w - 1);
     const int xu = (x == w - 1) ? x : x + 1;
     const int yu = (y == h - 1) ? y : y + 1;
     const float v0 = img[y * w + x], v1 = img[y * w + xu];
@@ -330,7 +336,7 @@ __device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restri
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
+        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.vcells, 0), P.grid());
         s_c[v][threadIdx.x] = cv;
         valid += (cv < 2.0f);
     }
@@ -589,7 +595,8 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
                                           const float4* __restrict__ planes,
                                           const float* __restrict__ costs, float* __restrict__ ca,
                                           float4* __restrict__ cplane, uint8_t* __restrict__ cflag,
-                                          int NA, float (*s_m)[3][kBlock], uint32_t& ncc) {
+                                          int NA, float (*s_m)[3][kBlock],
+                                          const float (*s_ab)[12], uint32_t& ncc) {
     const Active a = active_pixel<L>(P, tile, color);
     if (!a.ok) return;
     const bool lead = a.sub == 0;
@@ -624,36 +631,50 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     // wave's gathers in L1
     for (int v = 0; v < P.nv; ++v) {
         const ViewDesc& vd = P.views[v];
-        const QuadSrc src = quad_src<PR>(vd.cells, P.h * P.w, 0);
-        for (uint32_t mm = todo; mm; mm &= mm - 1u) {
-            const int k = __ffs(mm) - 1;
+        const QuadSrc src = quad_src<PR>(vd.cells, P.vcells, 0);
+        const float* ab = s_ab[v];  // the view's A (9) and b (3), LDS broadcast reads
+        // a wave-uniform hypothesis index (pixels without hypothesis k sit
+        // out that iteration): LDS and cost-table offsets stay scalar
+#pragma unroll 1
+        for (int k = 0; k < 9; ++k) {
+            if (!((todo >> k) & 1u)) continue;
             const float m[3] = {s_m[k][0][threadIdx.x], s_m[k][1][threadIdx.x], s_m[k][2][threadIdx.x]};
             float H[9];
-            homography(vd.A, vd.b, m, H);
-            const float cv = ncc_eval_mlp<NT, L, PR>(hs, H, src, P.h, P.w);
+            homography(ab, ab + 9, m, H);
+            const float cv = ncc_eval_mlp<NT, L, PR>(hs, H, src, P.grid());
             if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
     }
 }
 
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) void pm_candidates(const PMProblem* __restrict__ probs,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void pm_candidates(const PMProblem* __restrict__ probs,
                                                         const WorkTables* __restrict__ tabs,
                                                         int color, uint32_t* __restrict__ queue,
                                                         unsigned long long* work) {
     __shared__ int s_item;
     __shared__ float s_m[9][3][kBlock];
+    __shared__ float s_ab[kMaxViews][12];
     const WorkTable& t = tabs->t[WT_HALF];
     const int total = t.start[t.n];
     uint32_t ncc = 0;
-    int q = blockIdx.x % kXcds, tries = 0;
+    int q = blockIdx.x % kXcds, tries = 0, loaded = -1;
     for (;;) {
         const int item = next_item(queue, total, q, tries, s_item);
         if (item < 0) break;
         const int p = problem_of(t, item);
         const PMProblem& pr = probs[p];
+        if (p != loaded) {  // block-uniform; next_item's barrier retired the readers
+            for (int i = threadIdx.x; i < pr.P.nv * 12; i += kBlock) {
+                const ViewDesc& vd = pr.P.views[i / 12];
+                const int j = i % 12;
+                s_ab[i / 12][j] = j < 9 ? vd.A[j] : vd.b[j - 9];
+            }
+            __syncthreads();
+            loaded = p;
+        }
         pm_candidates_body<NT, L, PR>(pr.P, item - t.start[p], color, pr.planes, pr.costs, pr.ca,
-                                      pr.cplane, pr.cflag, pr.na, s_m, ncc);
+                                      pr.cplane, pr.cflag, pr.na, s_m, s_ab, ncc);
     }
     if (work) count_work(work + 2 * K_CANDIDATES, ncc, 0);
 }
@@ -1048,7 +1069,6 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
                                       const ViewDesc* s_views, uint16_t* units, float* res,
                                       uint32_t& ncc, uint32_t& geom) {
     static_assert(L == kLanes, "pm_refine is laid out for kLanes-wide groups");
-    using B = TapBlock<NT, L>;
     constexpr int NW = RefineSmem<NT>::NW;
     const RefCam& rc = P.rc;
     const int g = threadIdx.x / L, sub = threadIdx.x % L;
@@ -1108,12 +1128,7 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
             hs.wt[i] = S.hw[sl][sub][i];
             hs.wr[i] = S.hw[sl][sub][NW + i];
         }
-#pragma unroll
-        for (int r = 0; r < B::NR; ++r)
-            hs.tx[r] = (float)min(max(a.px - P.radius + B::row(sub, r) * P.inc, 0), P.w - 1);
-#pragma unroll
-        for (int q = 0; q < B::NC; ++q)
-            hs.ty[q] = (float)min(max(a.py - P.radius + B::col(sub, q) * P.inc, 0), P.h - 1);
+        hoist_taps<NT, L>(hs, a.px, a.py, P.h, P.w, P.radius, P.inc, sub);
         hs.inv_w = S.hp[sl][0];
         hs.mean_r = S.hp[sl][1];
         hs.var_r = S.hp[sl][2];
@@ -1121,7 +1136,7 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
         hs.px = fpx;
         hs.py = fpy;
         const ViewDesc& vd = s_views[v];
-        const QuadSrc all_views = quad_src<PR>(P.cells, (uint32_t)(P.nv * P.h * P.w), v * P.h * P.w);
+        const QuadSrc all_views = quad_src<PR>(P.cells, (uint32_t)(P.nv * P.vcells), v * P.vcells);
 #pragma unroll 1
         for (int k = 0; k < kRefine; ++k) {
             const float4 tp = S.tp[k][sl];
@@ -1129,7 +1144,7 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
             plane_m(rc, tp, m);
             float H[9];
             homography(vd.A, vd.b, m, H);
-            float x = ncc_eval_mlp<NT, L, PR>(hs, H, all_views, P.h, P.w);
+            float x = ncc_eval_mlp<NT, L, PR>(hs, H, all_views, P.grid());
             if (P.geom) x = x + 0.1f * geom_cost(rc, vd, tp, fpx, fpy);
             if (sub == 0) res[(k * kK2Pix + sl) * P.nv + v] = x;
         }
@@ -1201,7 +1216,7 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
 // Persistent over WT_HALF tiles. Dynamic LDS: ViewDesc[nvmax], uint16
 // units[64 * nvmax] (slot | view << 6), float results[kRefine][64][nvmax].
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) void pm_refine(const PMProblem* __restrict__ probs,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PR == PREC_F32 ? 3 : 4))) void pm_refine(const PMProblem* __restrict__ probs,
                                                    const WorkTables* __restrict__ tabs, int color,
                                                    int nvmax, uint32_t* __restrict__ queue,
                                                    unsigned long long* work) {
@@ -1374,7 +1389,7 @@ __global__ __launch_bounds__(kBlock) void pm_cost_batch(PMConst P, int n,
         const ViewDesc& vd = P.views[v];
         float H[9];
         homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.h * P.w, 0), P.h, P.w);
+        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.vcells, 0), P.grid());
         if (sub == 0) out[(size_t)q * P.nv + v] = cv;
     }
 }
@@ -1394,6 +1409,9 @@ bool valid_precision(int precision) {
 
 size_t cell_bytes(int precision) { return precision == GMVS_PREC_FP32 ? 16 : 8; }
 
+// cells of one view's padded grid
+size_t grid_cells(int h, int w) { return size_t(h + kPad) * size_t(w + kPad); }
+
 PMLayout pm_layout(int h, int w, int num_images, bool sweep, int precision) {
     const int nv = num_images - 1;
     const size_t hw = size_t(h) * w;
@@ -1403,7 +1421,7 @@ PMLayout pm_layout(int h, int w, int num_images, bool sweep, int precision) {
     L.views = o;
     o = align_up(o + sizeof(ViewDesc) * nv);
     L.quads = o;
-    o = align_up(o + cell_bytes(precision) * hw * nv);
+    o = align_up(o + cell_bytes(precision) * grid_cells(h, w) * nv);
     L.sel = o;
     if (sweep) {
         o = align_up(o + sizeof(uint32_t) * hw);
@@ -1438,9 +1456,11 @@ int check_cameras(int num_images, const gmvs_camera* cams) {
                    "num_images must be in [2, %d] (got %d)", kMaxViews + 1, num_images);
     const int h = cams[0].height, w = cams[0].width;
     GMVS_ARG_CHECK(h > 0 && w > 0, "reference image size must be positive (got %dx%d)", w, h);
-    // tap addresses are formed as y * w + x in FP32, exact up to 2^24 pixels
-    GMVS_ARG_CHECK((long long)h * w <= (1LL << 24), "image too large (%dx%d; at most 2^24 pixels)",
-                   w, h);
+    // tap addresses are formed as y * (w + kPad) + x in FP32 and in 24-bit
+    // integer products, exact while the padded grid has at most 2^24 cells
+    GMVS_ARG_CHECK((long long)(h + kPad) * (w + kPad) <= (1LL << 24),
+                   "image too large (%dx%d; (w + %d) * (h + %d) must be at most 2^24)", w, h, kPad,
+                   kPad);
     for (int i = 1; i < num_images; ++i)
         GMVS_ARG_CHECK(cams[i].height == h && cams[i].width == w,
                        "all images of a problem must share the reference size (view %d is %dx%d, "
@@ -1455,11 +1475,10 @@ int setup_views(int num_images, const gmvs_camera* cams, const float* const* ima
                 const float* const* depths, char* ws, const PMLayout& L, hipStream_t st) {
     const int h = cams[0].height, w = cams[0].width;
     const int nv = num_images - 1;
-    const size_t hw = size_t(h) * w;
     char* cells = ws + L.quads;
-    const size_t view_bytes = hw * Cell<PR>::kBytes;
+    const size_t view_bytes = grid_cells(h, w) * Cell<PR>::kBytes;
     ViewDesc* views = reinterpret_cast<ViewDesc*>(ws + L.views);
-    const int blocks = int((hw + 255) / 256);
+    const int blocks = int((grid_cells(h, w) + 255) / 256);
     for (int v = 0; v < nv; ++v) {
         GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_view, dim3(1), dim3(64), 0, st, cams[0], cams[v + 1],
                     (const void*)(cells + view_bytes * v), depths ? depths[v + 1] : nullptr, views + v);
@@ -1470,6 +1489,9 @@ int setup_views(int num_images, const gmvs_camera* cams, const float* const* ima
     }
     return GMVS_OK;
 }
+
+// Clamp-free tap indexing allowed (gmvs_set_fast_taps; results do not depend on it).
+std::atomic<int> g_fast_taps{1};
 
 PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, const ViewDesc* views,
                    const gmvs_params& prm, int radius) {
@@ -1492,6 +1514,9 @@ PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, co
     P.h = r.height;
     P.w = r.width;
     P.wh = (r.width + 1) / 2;
+    P.wp = r.width + kPad;
+    P.vcells = (int)grid_cells(r.height, r.width);
+    P.fast_taps = g_fast_taps.load();
     P.radius = radius;
     P.inc = prm.radius_increment;
     P.sigma_s = prm.sigma_spatial;
@@ -1827,6 +1852,13 @@ extern "C" int gmvs_set_blocks_per_cu(int32_t kernel, int32_t blocks) {
                     : kernel == GMVS_TUNE_CANDIDATES ? K_CANDIDATES
                                                      : K_REFINE;
     g_blocks_per_cu[kid].store(blocks);
+    return GMVS_OK;
+}
+
+extern "C" int gmvs_set_fast_taps(int32_t enable) {
+    clear_error();
+    GMVS_ARG_CHECK(enable == 0 || enable == 1, "enable must be 0 or 1 (got %d)", enable);
+    g_fast_taps.store(enable);
     return GMVS_OK;
 }
 

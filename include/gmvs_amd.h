@@ -165,6 +165,14 @@ enum {
 };
 int gmvs_set_blocks_per_cu(int32_t kernel, int32_t blocks);
 
+/* Tap indexing of the NCC gathers (no counterpart in the reference): 1 (the
+ * default) lets a wave whose patches all map inside the padded cell grid
+ * index its taps without clamping; 0 clamps every tap as the reference's
+ * software bilinear does (gmvs/src/patch_match.cu:16-31). Both forms give
+ * identical results; 0 exists to test that. Process-wide, read when a run
+ * is set up. */
+int gmvs_set_fast_taps(int32_t enable);
+
 /* Parity hook: V bilateral NCCs for n (pixel, plane) pairs -- the batched form
  * of ComputeMultiViewCostVector (gmvs/lib/src/patch_match_kernel.cu:571-588,
  * gmvs/src/patch_match.cu:503-522). xy: device [n*2] int32, planes: device

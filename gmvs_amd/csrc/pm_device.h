@@ -593,63 +593,43 @@ __device__ inline bool corners_in_grid(const Hoist<NT, L>& hs, const float H[9],
 //    are the same numbers; on the pad the value only depends on (v00, d10)
 //    or (v00, d01) of the edge cell, whose other terms are multiplied by
 //    exact zeros.
-template <int NT, int L, int PR, bool ALLFIRST>
-__device__ inline float ncc_eval_t(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
-                                   const SrcGrid& g) {
+// The taps of one lane in one of the two index forms, accumulated into the
+// moments. NG rows of gathers are in flight together.
+template <int NT, int L, int PR, int NG, bool FAST>
+__device__ inline void ncc_taps(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+                                const SrcGrid& g, float& S1, float& S2, float& X) {
     using B = TapBlock<NT, L>;
-    constexpr int NG = ALLFIRST ? B::NR : 1;  // rows whose gathers are in flight together
-    const float zc = fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8]));
-    const float rcz = __builtin_amdgcn_rcpf(zc);
-    const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
-    const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
-    if (cx >= (float)g.w || cx < 0.0f || cy >= (float)g.h || cy < 0.0f) return 2.0f;
-    // wave-uniform choice (the ballot covers the lanes still active here)
-    const bool fast =
-        g.fast && __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L>(hs, H, zc, g)) == 0;
     const float wm1 = (float)(g.w - 1), hm1 = (float)(g.h - 1), wpf = (float)g.wp;
-    float S1 = 0.f, S2 = 0.f, X = 0.f;
 #pragma unroll
     for (int r0 = 0; r0 < B::NR; r0 += NG) {
         Cell<PR> q[NG * B::NC];
         float fx[NG * B::NC], fy[NG * B::NC];
-        if (fast) {
 #pragma unroll
-            for (int rr = 0; rr < NG; ++rr) {
-                const int r = r0 + rr;
-                const float au = fmaf(H[0], hs.tx[r], H[2]);
-                const float av = fmaf(H[3], hs.tx[r], H[5]);
-                const float az = fmaf(H[6], hs.tx[r], H[8]);
+        for (int rr = 0; rr < NG; ++rr) {
+            const int r = r0 + rr;
+            const float au = fmaf(H[0], hs.tx[r], H[2]);
+            const float av = fmaf(H[3], hs.tx[r], H[5]);
+            const float az = fmaf(H[6], hs.tx[r], H[8]);
 #pragma unroll
-                for (int j = 0; j < B::NC; ++j) {
-                    const float rz = __builtin_amdgcn_rcpf(fmaf(H[7], hs.ty[j], az));
-                    const float x = fmaf(H[1], hs.ty[j], au) * rz;
-                    const float y = fmaf(H[4], hs.ty[j], av) * rz;
-                    const int t = rr * B::NC + j;
+            for (int j = 0; j < B::NC; ++j) {
+                const float rz = __builtin_amdgcn_rcpf(fmaf(H[7], hs.ty[j], az));
+                const float x = fmaf(H[1], hs.ty[j], au) * rz;
+                const float y = fmaf(H[4], hs.ty[j], av) * rz;
+                const int t = rr * B::NC + j;
+                if constexpr (FAST) {
                     fx[t] = __builtin_amdgcn_fractf(x);
                     fy[t] = __builtin_amdgcn_fractf(y);
-                    q[t].load(src, (int)(__umul24((unsigned)cvt_flr(y), (unsigned)g.wp) + (unsigned)cvt_flr(x)));
-                }
-            }
-        } else {
+                    q[t].load(src, (int)(__umul24((unsigned)cvt_flr(y), (unsigned)g.wp) +
+                                         (unsigned)cvt_flr(x)));
+                } else {
 #pragma clang fp contract(off)
-#pragma unroll
-            for (int rr = 0; rr < NG; ++rr) {
-                const int r = r0 + rr;
-                const float au = fmaf(H[0], hs.tx[r], H[2]);
-                const float av = fmaf(H[3], hs.tx[r], H[5]);
-                const float az = fmaf(H[6], hs.tx[r], H[8]);
-#pragma unroll
-                for (int j = 0; j < B::NC; ++j) {
-                    const float rz = __builtin_amdgcn_rcpf(fmaf(H[7], hs.ty[j], az));
-                    const float x = fmaf(H[1], hs.ty[j], au) * rz;
-                    const float y = fmaf(H[4], hs.ty[j], av) * rz;
                     // clamp(trunc(c), 0, n-1) of the reference equals
                     // clamp(floor(c), 0, n-1); any NaN/inf lands inside
-                    // [0, n-1]. No contraction: the fraction is formed from
-                    // the rounded x, as fract() is in the clamp-free form.
+                    // [0, n-1]. The fraction is the rounded x minus the
+                    // index (no contraction into x's product), as fract()
+                    // forms it in the clamp-free form.
                     const float xf = __builtin_amdgcn_fmed3f(floorf(x), 0.f, wm1);
                     const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
-                    const int t = rr * B::NC + j;
                     fx[t] = x - xf;
                     fy[t] = y - yf;
                     q[t].load(src, (int)(uint32_t)fmaf(yf, wpf, xf));
@@ -661,13 +641,55 @@ __device__ inline float ncc_eval_t(const Hoist<NT, L>& hs, const float H[9], con
 #pragma unroll
             for (int j = 0; j < B::NC; ++j) {
                 const int t = rr * B::NC + j;
+                // explicit FMAs: both index forms round identically
                 const float s = q[t].sample(fx[t], fy[t], hs.c);
-                const float ws = hs.wt[(r0 + rr) * B::NC + j] * s;
-                S1 += ws;
-                S2 = fmaf(ws, s, S2);
+                const float w = hs.wt[(r0 + rr) * B::NC + j];
+                S1 = fmaf(w, s, S1);
+                S2 = fmaf(w * s, s, S2);
                 X = fmaf(hs.wr[(r0 + rr) * B::NC + j], s, X);
             }
+        // keep the next rows' gathers behind this batch's use: hoisting them
+        // would hold more cells in VGPRs than the occupancy affords
+        if constexpr (NG < B::NR) __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// One bilateral NCC of the hoisted pixel against one source view. A tap is
+// one gather of a packed cell and its bilinear sample. ALLFIRST: a 4-lane
+// group's lane issues all its gathers before consuming the first (more loads
+// in flight per wave, for scattered footprints); else (and for narrower
+// groups, whose lanes own twice the taps) a patch row at a time. All lanes
+// of a group return the same cost.
+//
+// Tap index and fraction, two forms with identical results:
+//  - clamped (the reference's rule, src 16-31): index clamp(floor(x), 0,
+//    w-1), fraction x - index (negative left of the image: linear
+//    extrapolation; beyond the right edge the edge cell's zero differences
+//    repeat the edge value);
+//  - clamp-free, taken when every lane group of the wave passes
+//    corners_in_grid: floor(x) as the index into the padded grid and
+//    fract(x) = x - floor(x) as the fraction. Inside the image both forms
+//    are the same numbers; on the pad the value only depends on (v00, d10)
+//    or (v00, d01) of the edge cell, whose other terms are multiplied by
+//    exact zeros.
+template <int NT, int L, int PR, bool ALLFIRST>
+__device__ inline float ncc_eval_t(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+                                   const SrcGrid& g) {
+    using B = TapBlock<NT, L>;
+    constexpr int NG = (ALLFIRST && L == 4) ? B::NR : 1;
+    const float zc = fmaf(H[6], hs.px, fmaf(H[7], hs.py, H[8]));
+    const float rcz = __builtin_amdgcn_rcpf(zc);
+    const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
+    const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
+    if (cx >= (float)g.w || cx < 0.0f || cy >= (float)g.h || cy < 0.0f) return 2.0f;
+    // wave-uniform choice (the ballot covers the lanes still active here)
+    const bool fast =
+        g.fast && __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L>(hs, H, zc, g)) == 0;
+    float S1 = 0.f, S2 = 0.f, X = 0.f;
+    if (fast)
+        ncc_taps<NT, L, PR, NG, true>(hs, H, src, g, S1, S2, X);
+    else
+        ncc_taps<NT, L, PR, NG, false>(hs, H, src, g, S1, S2, X);
     S1 = group_sum<L>(S1);
     S2 = group_sum<L>(S2);
     X = group_sum<L>(X);

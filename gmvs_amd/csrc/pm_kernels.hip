@@ -36,8 +36,8 @@ namespace gmvs {
 constexpr int kBlock = 256;  // threads per block
 constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
 
-// Lane groups of kLanes lanes evaluate one pixel's NCCs together: a block
-// of 256 threads holds a tile of 64 pixels, TX x TY with TX = 1 << shift
+// Lane groups of L lanes evaluate one pixel's NCCs together: the block of an
+// NCC kernel (64 * L threads) holds a tile of 64 pixels, TX x TY with TX = 1 << shift
 // (TX counts active columns in the half-sweep kernels and pixel columns in
 // pm_init). The shape is chosen per problem (gmvs_pm_args.tile_cols).
 struct TileShape {
@@ -49,8 +49,16 @@ struct TileShape {
     }
 };
 
-// The NCC lane-group width used by the kernels.
-constexpr int kLanes = 4;
+// Pixels per NCC tile (one lane group each) and the block of an NCC kernel.
+constexpr int kTilePix = 64;
+template <int L>
+constexpr int ncc_block() { return kTilePix * L; }
+// The NCC lane-group width per sample precision: 4 lanes (9 taps each) for
+// FP32 cells, 2 lanes (18 taps each) for 16-bit cells. Wider groups keep
+// fewer cells in flight per lane (FP32 cells take 4 VGPRs); narrower groups
+// spend fewer instructions per NCC on the per-hypothesis set-up, the corner
+// test and the epilogue, which every lane of a group repeats.
+constexpr int lanes_for(int PR) { return PR == 0 ? 4 : 2; }
 constexpr int kNT = 6;  // taps per axis: patch 11, increment 2 (the reference defaults)
 
 struct PMConst {
@@ -275,7 +283,7 @@ __global__ void pm_pack_quads(const float* __restrict__ img, void* __restrict__ 
 template <int NT, int L, int PR>
 __device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restrict__ planes,
                                     float* __restrict__ costs, uint32_t* __restrict__ sel,
-                                    float (*s_c)[kBlock], uint32_t& ncc) {
+                                    float (*s_c)[kTilePix * L], uint32_t& ncc) {
     const int TX = P.tile.tx(), TY = P.tile.ty();
     const int tx = (P.w + TX - 1) / TX;
     const int slot = threadIdx.x / L, sub = threadIdx.x % L;
@@ -377,11 +385,11 @@ __device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restri
 }
 
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) void pm_init(const PMProblem* __restrict__ probs,
+__global__ __launch_bounds__(kTilePix * L) void pm_init(const PMProblem* __restrict__ probs,
                                                   const WorkTables* __restrict__ tabs,
                                                   uint32_t* __restrict__ queue,
                                                   unsigned long long* work) {
-    __shared__ float s_c[kMaxViews][kBlock];
+    __shared__ float s_c[kMaxViews][kTilePix * L];
     __shared__ int s_item;
     const WorkTable& t = tabs->t[WT_FULL];
     const int total = t.start[t.n];
@@ -595,7 +603,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
                                           const float4* __restrict__ planes,
                                           const float* __restrict__ costs, float* __restrict__ ca,
                                           float4* __restrict__ cplane, uint8_t* __restrict__ cflag,
-                                          int NA, float (*s_m)[3][kBlock],
+                                          int NA, float (*s_m)[3][kTilePix * L],
                                           const float (*s_ab)[12], uint32_t& ncc) {
     const Active a = active_pixel<L>(P, tile, color);
     if (!a.ok) return;
@@ -648,12 +656,12 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
 }
 
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void pm_candidates(const PMProblem* __restrict__ probs,
+__global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(4))) void pm_candidates(const PMProblem* __restrict__ probs,
                                                         const WorkTables* __restrict__ tabs,
                                                         int color, uint32_t* __restrict__ queue,
                                                         unsigned long long* work) {
     __shared__ int s_item;
-    __shared__ float s_m[9][3][kBlock];
+    __shared__ float s_m[9][3][kTilePix * L];
     __shared__ float s_ab[kMaxViews][12];
     const WorkTable& t = tabs->t[WT_HALF];
     const int total = t.start[t.n];
@@ -665,7 +673,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int p = problem_of(t, item);
         const PMProblem& pr = probs[p];
         if (p != loaded) {  // block-uniform; next_item's barrier retired the readers
-            for (int i = threadIdx.x; i < pr.P.nv * 12; i += kBlock) {
+            for (int i = threadIdx.x; i < pr.P.nv * 12; i += kTilePix * L) {
                 const ViewDesc& vd = pr.P.views[i / 12];
                 const int j = i % 12;
                 s_ab[i / 12][j] = j < 9 ? vd.A[j] : vd.b[j - 9];
@@ -693,7 +701,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 //                 block's 64 lane groups so that trip counts differ by at most
 //                 one whatever the per-pixel view counts are; then the
 //                 reference's sequential adoption in view order.
-constexpr int kK2Pix = kBlock / kLanes;  // active pixels per pm_refine block
+constexpr int kK2Pix = kTilePix;  // active pixels per pm_refine block
 constexpr int kRefine = 5;               // refinement hypotheses per pixel
 
 __device__ inline int argmin_last8(const float* c) {
@@ -1040,10 +1048,10 @@ __device__ inline Active active_slot(const PMConst& P, int tile, int color, int 
     return a;
 }
 
-template <int NT>
+template <int NT, int L>
 struct RefineSmem {
-    static constexpr int NW = TapBlock<NT, kLanes>::NR * TapBlock<NT, kLanes>::NC;
-    float hw[kK2Pix][kLanes][2 * NW];  // hoisted w and w * (r - c) per (pixel, lane)
+    static constexpr int NW = TapBlock<NT, L>::NR * TapBlock<NT, L>::NC;
+    float hw[kK2Pix][L][2 * NW];  // hoisted w and w * (r - c) per (pixel, lane)
     float hp[kK2Pix][4];               // 1 / sum w, mean_r, var_r, centre intensity
     float4 tp[kRefine][kK2Pix];        // refinement hypotheses
     int nunits;
@@ -1065,15 +1073,14 @@ template <int NT, int L, int PR>
 __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
                                       float4* __restrict__ planes, float* __restrict__ costs,
                                       const float4* __restrict__ rtp,
-                                      const uint32_t* __restrict__ rsw, int NA, RefineSmem<NT>& S,
+                                      const uint32_t* __restrict__ rsw, int NA, RefineSmem<NT, L>& S,
                                       const ViewDesc* s_views, uint16_t* units, float* res,
                                       uint32_t& ncc, uint32_t& geom) {
-    static_assert(L == kLanes, "pm_refine is laid out for kLanes-wide groups");
-    constexpr int NW = RefineSmem<NT>::NW;
+    constexpr int NW = RefineSmem<NT, L>::NW;
     const RefCam& rc = P.rc;
     const int g = threadIdx.x / L, sub = threadIdx.x % L;
     // the tile's refinement hypotheses
-    for (int i = threadIdx.x; i < kK2Pix * kRefine; i += kBlock) {
+    for (int i = threadIdx.x; i < kK2Pix * kRefine; i += kTilePix * L) {
         const int sl = i % kK2Pix, f = i / kK2Pix;
         const Active a = active_slot(P, tile, color, sl);
         S.tp[f][sl] = a.ok ? rtp[(size_t)f * NA + a.aidx] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1216,11 +1223,11 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
 // Persistent over WT_HALF tiles. Dynamic LDS: ViewDesc[nvmax], uint16
 // units[64 * nvmax] (slot | view << 6), float results[kRefine][64][nvmax].
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PR == PREC_F32 ? 3 : 4))) void pm_refine(const PMProblem* __restrict__ probs,
+__global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(PR == PREC_F32 ? 3 : 4))) void pm_refine(const PMProblem* __restrict__ probs,
                                                    const WorkTables* __restrict__ tabs, int color,
                                                    int nvmax, uint32_t* __restrict__ queue,
                                                    unsigned long long* work) {
-    __shared__ RefineSmem<NT> S;
+    __shared__ RefineSmem<NT, L> S;
     __shared__ int s_item;
     extern __shared__ __align__(16) unsigned char ref_dyn[];
     ViewDesc* s_views = reinterpret_cast<ViewDesc*>(ref_dyn);
@@ -1239,7 +1246,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PR == PR
             const int n = pr.P.nv * (int)(sizeof(ViewDesc) / 4);
             const uint32_t* src = reinterpret_cast<const uint32_t*>(pr.P.views);
             uint32_t* dst = reinterpret_cast<uint32_t*>(s_views);
-            for (int i = threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+            for (int i = threadIdx.x; i < n; i += kTilePix * L) dst[i] = src[i];
             loaded = p;
         }
         pm_refine_body<NT, L, PR>(pr.P, item - t.start[p], color, pr.planes, pr.costs, pr.rtp,
@@ -1368,7 +1375,7 @@ __global__ void pm_export(const PMProblem* __restrict__ probs, const WorkTables*
 
 // Parity hook: V NCCs for a list of (pixel, plane) pairs.
 template <int NT, int L, int PR>
-__global__ __launch_bounds__(kBlock) void pm_cost_batch(PMConst P, int n,
+__global__ __launch_bounds__(kTilePix * L) void pm_cost_batch(PMConst P, int n,
                                                         const int32_t* __restrict__ xy,
                                                         const float4* __restrict__ planes,
                                                         float* __restrict__ out) {
@@ -1615,7 +1622,7 @@ std::atomic<int> g_blocks_per_cu[K_NUM];
 // Blocks of a persistent kernel on the current device: all CUs x the
 // occupancy at this block size and dynamic LDS (cached per kernel), or the
 // tuned blocks per CU when that is lower.
-int persistent_grid(const void* kernel, size_t dyn_lds, int items, int kid) {
+int persistent_grid(const void* kernel, int block, size_t dyn_lds, int items, int kid) {
     static std::mutex mu;
     static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
     int per_cu = 0;
@@ -1626,7 +1633,7 @@ int persistent_grid(const void* kernel, size_t dyn_lds, int items, int kid) {
     }
     if (per_cu == 0) {
         int bl = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bl, kernel, kBlock, dyn_lds) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bl, kernel, block, dyn_lds) != hipSuccess ||
             bl <= 0)
             bl = 1;
         int dev = 0, cus = 0;
@@ -1716,25 +1723,29 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
     const int n_act = tabs.t[WT_ACTIVE].start[n], n_pix = tabs.t[WT_PIXELS].start[n];
     const size_t sel_lds = sizeof(ViewDesc) * (size_t)nvmax + sizeof(float) * (size_t)nvmax * kBlock;
     const size_t ref_lds = refine_dyn_bytes(nvmax);
-    auto k_init = pm_init<kNT, kLanes, PR>;
-    auto k_cand = pm_candidates<kNT, kLanes, PR>;
-    auto k_ref = pm_refine<kNT, kLanes, PR>;
-    const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), 0, n_full, K_INIT));
-    const dim3 g_cand(persistent_grid(reinterpret_cast<const void*>(k_cand), 0, n_half, K_CANDIDATES));
-    const dim3 g_ref(persistent_grid(reinterpret_cast<const void*>(k_ref), ref_lds, n_half, K_REFINE));
+    constexpr int L = lanes_for(PR);
+    constexpr int nb = ncc_block<L>();
+    auto k_init = pm_init<kNT, L, PR>;
+    auto k_cand = pm_candidates<kNT, L, PR>;
+    auto k_ref = pm_refine<kNT, L, PR>;
+    const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), nb, 0, n_full, K_INIT));
+    const dim3 g_cand(
+        persistent_grid(reinterpret_cast<const void*>(k_cand), nb, 0, n_half, K_CANDIDATES));
+    const dim3 g_ref(
+        persistent_grid(reinterpret_cast<const void*>(k_ref), nb, ref_lds, n_half, K_REFINE));
 
-    GMVS_LAUNCH(K_INIT, st, k_init, g_init, dim3(kBlock), 0, st, d_probs, d_tabs, queue, work);
+    GMVS_LAUNCH(K_INIT, st, k_init, g_init, dim3(nb), 0, st, d_probs, d_tabs, queue, work);
     GMVS_LAUNCH_CHECK();
     for (int it = 0; it < iters; ++it) {
         for (int color = 0; color < 2; ++color) {
             const int slot = 1 + 2 * (2 * it + color);
-            GMVS_LAUNCH(K_CANDIDATES, st, k_cand, g_cand, dim3(kBlock), 0, st, d_probs, d_tabs, color,
+            GMVS_LAUNCH(K_CANDIDATES, st, k_cand, g_cand, dim3(nb), 0, st, d_probs, d_tabs, color,
                         queue + kXcds * slot, work);
             GMVS_LAUNCH_CHECK();
             GMVS_LAUNCH(K_SELECT, st, pm_select, dim3(n_act), dim3(kBlock), sel_lds, st, d_probs, d_tabs,
                         color, it, nvmax, work);
             GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_REFINE, st, k_ref, g_ref, dim3(kBlock), ref_lds, st, d_probs, d_tabs, color,
+            GMVS_LAUNCH(K_REFINE, st, k_ref, g_ref, dim3(nb), ref_lds, st, d_probs, d_tabs, color,
                         nvmax, queue + kXcds * (slot + 1), work);
             GMVS_LAUNCH_CHECK();
         }
@@ -1770,7 +1781,9 @@ int pm_run_batch(const gmvs_pm_args* args, int n, void* workspace, size_t worksp
         int dev = 0, lds = 0;
         GMVS_HIP_CHECK(hipGetDevice(&dev));
         GMVS_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = sizeof(RefineSmem<kNT>) + refine_dyn_bytes(nvmax);
+        const size_t need = std::max(sizeof(RefineSmem<kNT, lanes_for(GMVS_PREC_FP32)>),
+                                     sizeof(RefineSmem<kNT, lanes_for(GMVS_PREC_FP16)>)) +
+                            refine_dyn_bytes(nvmax);
         if (need > (size_t)lds) {
             set_error("%d source views need %zu bytes of LDS per block; the device allows %d",
                       nvmax, need, lds);
@@ -1805,8 +1818,10 @@ int cost_batch(int num_images, const gmvs_camera* cameras, const float* const* i
     PMConst P = make_const(num_images, cameras, images[0],
                            reinterpret_cast<const ViewDesc*>(ws + L.views), params, radius);
     P.cells = ws + L.quads;
-    GMVS_LAUNCH(K_COST_BATCH, st, (pm_cost_batch<kNT, kLanes, PR>),
-                dim3((n * kLanes + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, n, xy,
+    constexpr int LG = lanes_for(PR);
+    constexpr int nb = ncc_block<LG>();
+    GMVS_LAUNCH(K_COST_BATCH, st, (pm_cost_batch<kNT, LG, PR>), dim3((n * LG + nb - 1) / nb),
+                dim3(nb), 0, st, P, n, xy,
                 reinterpret_cast<const float4*>(planes), costs);
     GMVS_LAUNCH_CHECK();
     return GMVS_OK;

@@ -113,6 +113,7 @@ EXPORTS = (
     "gmvs_pm_batch_workspace_bytes",
     "gmvs_pm_run_batch",
     "gmvs_set_blocks_per_cu",
+    "gmvs_set_lane_width",
     "gmvs_set_fast_taps",
     "gmvs_cost_workspace_bytes",
     "gmvs_cost_batch",
@@ -189,6 +190,7 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_pm_batch_workspace_bytes.restype = sz
     lib.gmvs_pm_run_batch.argtypes = [ctypes.POINTER(CPMArgs), i32, vp, sz, vp]
     lib.gmvs_set_blocks_per_cu.argtypes = [i32, i32]
+    lib.gmvs_set_lane_width.argtypes = [i32, i32]
     lib.gmvs_set_fast_taps.argtypes = [i32]
     lib.gmvs_cost_workspace_bytes.argtypes = [i32, i32, i32, i32]
     lib.gmvs_cost_workspace_bytes.restype = sz
@@ -272,3 +274,8 @@ def camera_array(cams: Sequence[CCamera]):
 def set_fast_taps(enable: bool) -> None:
     """Clamp-free NCC tap indexing on (default) or off; results are identical."""
     check("gmvs_set_fast_taps", lib.gmvs_set_fast_taps(1 if enable else 0))
+
+
+def set_lane_width(kernel: int, lanes: int) -> None:
+    """Lanes per pixel (2 or 4; 0 = default) of a PatchMatch NCC kernel (TUNE_*)."""
+    check("gmvs_set_lane_width", lib.gmvs_set_lane_width(kernel, lanes))

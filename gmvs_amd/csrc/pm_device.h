@@ -446,9 +446,10 @@ constexpr int kBufWord3 = 0x00020000;
 //                differences are stored exactly, so 8-bit images give the
 //                FP32 path's results bit for bit with half its gather bytes;
 //   PREC_BF16 -- the cell in BF16, 8 bytes, and the bilinear sample s - c as
-//                two BF16 dot products with FP32 accumulation:
-//                dot((v00, v01-v00), (1, fx)) + dot((v10-v00, d11), (fy, fx*fy))
-//                - c. The weights carry BF16 rounding (8-bit mantissa).
+//                two BF16 dot products with FP32 accumulation and one FP32
+//                FMA: dot((v00, v01-v00), (1, fx)) - c
+//                + fy * dot((v10-v00, d11), (1, fx)). fx carries BF16
+//                rounding (8-bit mantissa); fy stays FP32.
 // The NCC moments are accumulated in FP32 about the centre intensity in
 // every mode.
 enum : int { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
@@ -497,10 +498,12 @@ struct Cell<PREC_BF16> {
     }
     __device__ float sample(float fx, float fy, float c) const {
         const int q0 = q.x, q1 = q.y;  // see Cell<PREC_F16>::sample
+        // both rows of the cell against one weight pair (1, fx): the row
+        // values v00 + fx d01 and d10 + fx d11, then the FP32 row blend by fy
         const bf16x2 a = __builtin_convertvector((f32x2){1.0f, fx}, bf16x2);
-        const bf16x2 b = __builtin_convertvector((f32x2){fy, fx * fy}, bf16x2);
-        const float lo = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q1), b, -c, false);
-        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q0), a, lo, false);
+        const float top = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q0), a, -c, false);
+        const float dyv = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q1), a, 0.0f, false);
+        return fmaf(fy, dyv, top);
     }
 };
 

@@ -53,12 +53,19 @@ struct TileShape {
 constexpr int kTilePix = 64;
 template <int L>
 constexpr int ncc_block() { return kTilePix * L; }
-// The NCC lane-group width per sample precision: 4 lanes (9 taps each) for
-// FP32 cells, 2 lanes (18 taps each) for 16-bit cells. Wider groups keep
-// fewer cells in flight per lane (FP32 cells take 4 VGPRs); narrower groups
-// spend fewer instructions per NCC on the per-hypothesis set-up, the corner
-// test and the epilogue, which every lane of a group repeats.
-constexpr int lanes_for(int PR) { return PR == 0 ? 4 : 2; }
+// NCC lane-group widths: 2 lanes (18 taps each) or 4 lanes (9 taps each),
+// chosen per kernel at run time (gmvs_set_lane_width; defaults below).
+// Narrower groups spend fewer instructions per NCC on the per-hypothesis
+// set-up, the corner test and the epilogue, which every lane of a group
+// repeats; wider groups keep fewer taps per lane and more lanes per pixel,
+// which suits the latency-bound refinement kernel (scattered random-depth
+// footprints). Measured on the 10-view 1600x1200 ACMP step, all three
+// precisions: candidates and init fastest with 2 lanes (-10% / -12% against
+// 4), refinement with 4 (-3% to -10% against 2).
+inline int default_lanes(int kid, int precision) {
+    (void)precision;
+    return kid == K_REFINE ? 4 : 2;
+}
 constexpr int kNT = 6;  // taps per axis: patch 11, increment 2 (the reference defaults)
 
 struct PMConst {
@@ -1619,6 +1626,14 @@ int check_args(const gmvs_pm_args& A, int* radius) {
 // gmvs_set_blocks_per_cu; the autotuner picks them by timing.
 std::atomic<int> g_blocks_per_cu[K_NUM];
 
+// Lane-group width per NCC kernel (0 = default_lanes), set by gmvs_set_lane_width.
+std::atomic<int> g_lanes[K_NUM];
+
+int lanes_of(int kid, int precision) {
+    const int l = g_lanes[kid].load();
+    return l ? l : default_lanes(kid, precision);
+}
+
 // Blocks of a persistent kernel on the current device: all CUs x the
 // occupancy at this block size and dynamic LDS (cached per kernel), or the
 // tuned blocks per CU when that is lower.
@@ -1723,29 +1738,30 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
     const int n_act = tabs.t[WT_ACTIVE].start[n], n_pix = tabs.t[WT_PIXELS].start[n];
     const size_t sel_lds = sizeof(ViewDesc) * (size_t)nvmax + sizeof(float) * (size_t)nvmax * kBlock;
     const size_t ref_lds = refine_dyn_bytes(nvmax);
-    constexpr int L = lanes_for(PR);
-    constexpr int nb = ncc_block<L>();
-    auto k_init = pm_init<kNT, L, PR>;
-    auto k_cand = pm_candidates<kNT, L, PR>;
-    auto k_ref = pm_refine<kNT, L, PR>;
-    const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), nb, 0, n_full, K_INIT));
+    // lane-group width per kernel: block = 64 pixels x L lanes
+    const int li = lanes_of(K_INIT, PR), lc = lanes_of(K_CANDIDATES, PR), lr = lanes_of(K_REFINE, PR);
+    auto k_init = li == 2 ? pm_init<kNT, 2, PR> : pm_init<kNT, 4, PR>;
+    auto k_cand = lc == 2 ? pm_candidates<kNT, 2, PR> : pm_candidates<kNT, 4, PR>;
+    auto k_ref = lr == 2 ? pm_refine<kNT, 2, PR> : pm_refine<kNT, 4, PR>;
+    const int bi = kTilePix * li, bc = kTilePix * lc, br = kTilePix * lr;
+    const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), bi, 0, n_full, K_INIT));
     const dim3 g_cand(
-        persistent_grid(reinterpret_cast<const void*>(k_cand), nb, 0, n_half, K_CANDIDATES));
+        persistent_grid(reinterpret_cast<const void*>(k_cand), bc, 0, n_half, K_CANDIDATES));
     const dim3 g_ref(
-        persistent_grid(reinterpret_cast<const void*>(k_ref), nb, ref_lds, n_half, K_REFINE));
+        persistent_grid(reinterpret_cast<const void*>(k_ref), br, ref_lds, n_half, K_REFINE));
 
-    GMVS_LAUNCH(K_INIT, st, k_init, g_init, dim3(nb), 0, st, d_probs, d_tabs, queue, work);
+    GMVS_LAUNCH(K_INIT, st, k_init, g_init, dim3(bi), 0, st, d_probs, d_tabs, queue, work);
     GMVS_LAUNCH_CHECK();
     for (int it = 0; it < iters; ++it) {
         for (int color = 0; color < 2; ++color) {
             const int slot = 1 + 2 * (2 * it + color);
-            GMVS_LAUNCH(K_CANDIDATES, st, k_cand, g_cand, dim3(nb), 0, st, d_probs, d_tabs, color,
+            GMVS_LAUNCH(K_CANDIDATES, st, k_cand, g_cand, dim3(bc), 0, st, d_probs, d_tabs, color,
                         queue + kXcds * slot, work);
             GMVS_LAUNCH_CHECK();
             GMVS_LAUNCH(K_SELECT, st, pm_select, dim3(n_act), dim3(kBlock), sel_lds, st, d_probs, d_tabs,
                         color, it, nvmax, work);
             GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_REFINE, st, k_ref, g_ref, dim3(nb), ref_lds, st, d_probs, d_tabs, color,
+            GMVS_LAUNCH(K_REFINE, st, k_ref, g_ref, dim3(br), ref_lds, st, d_probs, d_tabs, color,
                         nvmax, queue + kXcds * (slot + 1), work);
             GMVS_LAUNCH_CHECK();
         }
@@ -1781,8 +1797,7 @@ int pm_run_batch(const gmvs_pm_args* args, int n, void* workspace, size_t worksp
         int dev = 0, lds = 0;
         GMVS_HIP_CHECK(hipGetDevice(&dev));
         GMVS_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = std::max(sizeof(RefineSmem<kNT, lanes_for(GMVS_PREC_FP32)>),
-                                     sizeof(RefineSmem<kNT, lanes_for(GMVS_PREC_FP16)>)) +
+        const size_t need = std::max(sizeof(RefineSmem<kNT, 2>), sizeof(RefineSmem<kNT, 4>)) +
                             refine_dyn_bytes(nvmax);
         if (need > (size_t)lds) {
             set_error("%d source views need %zu bytes of LDS per block; the device allows %d",
@@ -1818,10 +1833,11 @@ int cost_batch(int num_images, const gmvs_camera* cameras, const float* const* i
     PMConst P = make_const(num_images, cameras, images[0],
                            reinterpret_cast<const ViewDesc*>(ws + L.views), params, radius);
     P.cells = ws + L.quads;
-    constexpr int LG = lanes_for(PR);
-    constexpr int nb = ncc_block<LG>();
-    GMVS_LAUNCH(K_COST_BATCH, st, (pm_cost_batch<kNT, LG, PR>), dim3((n * LG + nb - 1) / nb),
-                dim3(nb), 0, st, P, n, xy,
+    // the candidate kernel's lane-group width (the costs of a pass's hypotheses)
+    const int lg = lanes_of(K_CANDIDATES, PR);
+    const int nb = kTilePix * lg;
+    auto k = lg == 2 ? pm_cost_batch<kNT, 2, PR> : pm_cost_batch<kNT, 4, PR>;
+    GMVS_LAUNCH(K_COST_BATCH, st, k, dim3((n * lg + nb - 1) / nb), dim3(nb), 0, st, P, n, xy,
                 reinterpret_cast<const float4*>(planes), costs);
     GMVS_LAUNCH_CHECK();
     return GMVS_OK;
@@ -1867,6 +1883,19 @@ extern "C" int gmvs_set_blocks_per_cu(int32_t kernel, int32_t blocks) {
                     : kernel == GMVS_TUNE_CANDIDATES ? K_CANDIDATES
                                                      : K_REFINE;
     g_blocks_per_cu[kid].store(blocks);
+    return GMVS_OK;
+}
+
+extern "C" int gmvs_set_lane_width(int32_t kernel, int32_t lanes) {
+    clear_error();
+    GMVS_ARG_CHECK(kernel == GMVS_TUNE_INIT || kernel == GMVS_TUNE_CANDIDATES ||
+                       kernel == GMVS_TUNE_REFINE,
+                   "unknown tunable kernel %d", kernel);
+    GMVS_ARG_CHECK(lanes == 0 || lanes == 2 || lanes == 4, "lanes must be 0, 2 or 4 (got %d)", lanes);
+    const int kid = kernel == GMVS_TUNE_INIT ? K_INIT
+                    : kernel == GMVS_TUNE_CANDIDATES ? K_CANDIDATES
+                                                     : K_REFINE;
+    g_lanes[kid].store(lanes);
     return GMVS_OK;
 }
 

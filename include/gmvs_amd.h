@@ -165,6 +165,13 @@ enum {
 };
 int gmvs_set_blocks_per_cu(int32_t kernel, int32_t blocks);
 
+/* Launch tuning (no counterpart in the reference): lanes per pixel of the
+ * NCC lane groups of a persistent PatchMatch kernel (GMVS_TUNE_*): 2 or 4,
+ * 0 = the default for the run's precision. The block holds 64 pixels x lanes
+ * threads. Changes the order of the FP32 NCC sums (rounding-level cost
+ * differences), not the algorithm. Process-wide; picked by gmvs_amd.autotune. */
+int gmvs_set_lane_width(int32_t kernel, int32_t lanes);
+
 /* Tap indexing of the NCC gathers (no counterpart in the reference): 1 (the
  * default) lets a wave whose patches all map inside the padded cell grid
  * index its taps without clamping; 0 clamps every tap as the reference's

@@ -5,11 +5,11 @@ under the same Philox stream, on the 5-view 320x240 scene (4 iterations).
 The cells hold uint8 intensities and their differences exactly in both
 formats. FP16 samples are FP32 arithmetic on the widened cell values
 (v_fma_mix_f32), so FP16 carries the FP32 path's error (and its bits, see
-test_gpu_taps.py). BF16 samples are BF16 dot products: the bilinear weights
-(fx, fy, fx*fy) are rounded to 8 significant bits inside them; the NCC sums
-stay FP32. Small cost errors shift the Monte-Carlo view selection and the
-adoption order, so the joint depth + 1-degree-normal agreement drops faster
-than the depth error.
+test_gpu_taps.py). BF16 samples are BF16 dot products of both cell rows
+with the weight pair (1, fx), fx rounded to 8 significant bits, blended by
+an FP32 fy; the NCC sums stay FP32. Small cost errors shift the Monte-Carlo
+view selection and the adoption order, so the joint depth + 1-degree-normal
+agreement drops faster than the depth error.
 """
 import numpy as np
 import pytest

@@ -261,6 +261,7 @@ def run_scenes4(args) -> int:
     from gmvs_amd import _native as N
     from gmvs_amd import src as S
     from gmvs_amd.autotune import autotune
+    from gmvs_amd.autotune import reset as autotune_reset
     from gmvs_amd.pipeline import cameras_from_synthetic
     from gmvs_amd.synthetic import make_scene
 
@@ -309,7 +310,8 @@ def run_scenes4(args) -> int:
     for name, prec, tune in runs:
         batch = [dict(p, precision=prec) for p in problems]
         tuned = autotune(batch) if tune else {"tile_cols": 8, "blocks_candidates": 0,
-                                              "blocks_refine": 0}
+                                              "blocks_refine": 0, "lanes_candidates": 0,
+                                              "lanes_refine": 0}
         batch = [dict(p, tile_cols=tuned["tile_cols"]) for p in batch]
         ws = S._Workspace()
         stream = torch.cuda.Stream()
@@ -332,10 +334,8 @@ def run_scenes4(args) -> int:
         log(f"{name} batched graph: {rate:.1f} Mpix-hyp/s ({rate / plain_rate:.3f}x), tuned {tuned}")
         del g, ws
         torch.cuda.empty_cache()
-        N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_CANDIDATES, 0))
-        N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_REFINE, 0))
-    N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_CANDIDATES, 0))
-    N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(N.TUNE_REFINE, 0))
+        autotune_reset()
+    autotune_reset()
     best = results["bf16_batched_graph"]["value"]
     line = {
         "metric": "PatchMatch Mpix-hyp/s + fusion Mpts/s; batched BF16 vs plain FP32 at fixed err",

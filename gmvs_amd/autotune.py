@@ -1,11 +1,23 @@
 """Autotuner of the PatchMatch launch shape.
 
-Two knobs change how the work is laid out on the GPU but never the results:
+Three knobs change how the work is laid out on the GPU:
 
 * the tile shape of the 64-pixel work tiles, per problem (`tile_cols` of
   gmvs_pm_args: 4, 8, 16 or 32 columns);
 * the blocks per CU of each persistent kernel (gmvs_set_blocks_per_cu: 0 =
-  as many as fit).
+  as many as fit);
+* the lanes per pixel of the NCC lane groups of the candidate and
+  refinement kernels (gmvs_set_lane_width: 2 or 4; 0 = the library's
+  default).
+
+The first two never change results; the lane width changes the order of the
+FP32 NCC sums (rounding-level cost differences). The lane width is not
+searched by default: the short trials start from random planes, whose
+scattered gathers favour 4 lanes, while the converged iterations that
+dominate a full pass favour 2 (the library default, measured on full
+passes); pass `lanes=(0, 2, 4)` to search it anyway. The blocks-per-CU
+candidates are given as waves per SIMD and converted with the kernel's
+block size (64 pixels x lanes).
 
 `autotune(problems)` times short plain passes (random initialisation, one
 iteration) of the given batch on scratch state for each setting and keeps
@@ -22,7 +34,10 @@ from . import _native as N
 from . import src as S
 
 TILE_COLS = (4, 8, 16, 32)
-BLOCKS = (0, 3, 2)
+WAVES_PER_SIMD = (0, 3, 2)  # 0 = as many as fit
+LANES = (0,)
+# the library's default lane widths (gmvs_set_lane_width(k, 0))
+DEFAULT_LANES = {N.TUNE_CANDIDATES: 2, N.TUNE_REFINE: 4}
 
 _cache: Dict[Tuple, dict] = {}
 
@@ -79,13 +94,21 @@ def _pick(times: Dict[int, float], default: int, margin: float) -> int:
     return best if times[best] < times[default] * (1.0 - margin) else default
 
 
+def _blocks(waves_per_simd: int, lanes: int) -> int:
+    """Blocks per CU for a waves-per-SIMD target (4 SIMDs per CU, blocks of
+    64 pixels x lanes threads = lanes waves); 0 stays 0."""
+    return 0 if waves_per_simd == 0 else max(1, 4 * waves_per_simd // lanes)
+
+
 def autotune(problems: Sequence[dict], iterations: int = 2, tile_cols: Sequence[int] = TILE_COLS,
-             blocks: Sequence[int] = BLOCKS, apply: bool = True, margin: float = 0.03) -> dict:
-    """Best {'tile_cols', 'blocks_candidates', 'blocks_refine'} for a batch of
-    make_pm_args keyword dicts, keeping the defaults (8 x 8 tiles, full
-    occupancy) unless another setting is faster by more than `margin`; with
-    apply=True the blocks per CU are set process-wide. The caller's state
-    buffers are not touched."""
+             waves: Sequence[int] = WAVES_PER_SIMD, lanes: Sequence[int] = LANES,
+             apply: bool = True, margin: float = 0.03) -> dict:
+    """Best {'tile_cols', 'blocks_candidates', 'blocks_refine',
+    'lanes_candidates', 'lanes_refine'} for a batch of make_pm_args keyword
+    dicts, keeping the defaults (8 x 8 tiles, full occupancy, the library's
+    lane widths) unless another setting is faster by more than `margin`; with
+    apply=True the blocks per CU and lane widths are set process-wide. The
+    caller's state buffers are not touched."""
     key = _key(problems)
     if key not in _cache:
         ws = S._Workspace()
@@ -96,11 +119,20 @@ def autotune(problems: Sequence[dict], iterations: int = 2, tile_cols: Sequence[
         for name, tune_id, kname in (("blocks_candidates", N.TUNE_CANDIDATES, "pm_candidates"),
                                      ("blocks_refine", N.TUNE_REFINE, "pm_refine")):
             times = {}
-            for b in blocks:
+            for wv in waves:
+                b = _blocks(wv, DEFAULT_LANES[tune_id])
                 N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(tune_id, b))
                 times[b] = _time(trial, ws, kname)
-            choice[name] = _pick(times, 0 if 0 in times else blocks[0], margin)
+            choice[name] = _pick(times, 0 if 0 in times else min(times), margin)
             N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(tune_id, 0))
+        for name, tune_id, kname in (("lanes_candidates", N.TUNE_CANDIDATES, "pm_candidates"),
+                                     ("lanes_refine", N.TUNE_REFINE, "pm_refine")):
+            times = {}
+            for ln in lanes:
+                N.set_lane_width(tune_id, ln)
+                times[ln] = _time(trial, ws, kname) if len(lanes) > 1 else 0.0
+            choice[name] = _pick(times, 0 if 0 in times else lanes[0], margin)
+            N.set_lane_width(tune_id, 0)
         _cache[key] = choice
     choice = _cache[key]
     if apply:
@@ -108,4 +140,13 @@ def autotune(problems: Sequence[dict], iterations: int = 2, tile_cols: Sequence[
                 N.lib.gmvs_set_blocks_per_cu(N.TUNE_CANDIDATES, choice["blocks_candidates"]))
         N.check("gmvs_set_blocks_per_cu",
                 N.lib.gmvs_set_blocks_per_cu(N.TUNE_REFINE, choice["blocks_refine"]))
+        N.set_lane_width(N.TUNE_CANDIDATES, choice.get("lanes_candidates", 0))
+        N.set_lane_width(N.TUNE_REFINE, choice.get("lanes_refine", 0))
     return dict(choice)
+
+
+def reset() -> None:
+    """Back to the library defaults for every process-wide knob autotune sets."""
+    for tune_id in (N.TUNE_CANDIDATES, N.TUNE_REFINE):
+        N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(tune_id, 0))
+        N.set_lane_width(tune_id, 0)

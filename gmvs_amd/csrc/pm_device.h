@@ -350,9 +350,25 @@ struct Hoist {
     float px, py;
 };
 
-// Fills the tap coordinates of a lane (tx, ty, kx, ky).
+// The hoisted pixel with its weights read from LDS at each use (pm_refine:
+// the tile's weights already sit in LDS, and not holding them in VGPRs
+// leaves room for more waves).
 template <int NT, int L>
-__device__ inline void hoist_taps(Hoist<NT, L>& hs, int px, int py, int h, int w, int radius,
+struct HoistRef {
+    using B = TapBlock<NT, L>;
+    const float* wt;  // LDS, [B::NR * B::NC]
+    const float* wr;
+    float tx[B::NR];
+    float ty[B::NC];
+    float kx[B::NKX];
+    float ky[B::NKY];
+    float inv_w, mean_r, var_r, c;
+    float px, py;
+};
+
+// Fills the tap coordinates of a lane (tx, ty, kx, ky).
+template <int NT, int L, class HS>
+__device__ inline void hoist_taps(HS& hs, int px, int py, int h, int w, int radius,
                                   int inc, int sub) {
     using B = TapBlock<NT, L>;
 #pragma unroll
@@ -388,7 +404,7 @@ __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref
     hs.c = c;
     const float ss = 2.0f * sigma_spatial * sigma_spatial;
     const float sc = 2.0f * sigma_color * sigma_color;
-    hoist_taps<NT, L>(hs, px, py, h, w, radius, inc, sub);
+    hoist_taps<NT, L, Hoist<NT, L>>(hs, px, py, h, w, radius, inc, sub);
     float W = 0.f, R1 = 0.f, R2 = 0.f;
 #pragma unroll
     for (int r = 0; r < B::NR; ++r) {
@@ -555,9 +571,8 @@ __device__ inline int cvt_flr(float x) {
 // inside the padded grid and none is left of or above the image, so the
 // taps need no clamping: the padded grid gives them the reference's clamped
 // values. Otherwise (and for NaN) false.
-template <int NT, int L>
-__device__ inline bool corners_in_grid(const Hoist<NT, L>& hs, const float H[9], float zc,
-                                       const SrcGrid& g) {
+template <int NT, int L, class HS>
+__device__ inline bool corners_in_grid(const HS& hs, const float H[9], float zc, const SrcGrid& g) {
     using B = TapBlock<NT, L>;
     constexpr float m = 0.0625f;
     const float xmax = (float)(g.w + kPad) - m, ymax = (float)(g.h + kPad) - m;
@@ -598,8 +613,8 @@ __device__ inline bool corners_in_grid(const Hoist<NT, L>& hs, const float H[9],
 //    exact zeros.
 // The taps of one lane in one of the two index forms, accumulated into the
 // moments. NG rows of gathers are in flight together.
-template <int NT, int L, int PR, int NG, bool FAST>
-__device__ inline void ncc_taps(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+template <int NT, int L, int PR, int NG, bool FAST, class HS>
+__device__ inline void ncc_taps(const HS& hs, const float H[9], const QuadSrc& src,
                                 const SrcGrid& g, float& S1, float& S2, float& X) {
     using B = TapBlock<NT, L>;
     const float wm1 = (float)(g.w - 1), hm1 = (float)(g.h - 1), wpf = (float)g.wp;
@@ -675,8 +690,8 @@ __device__ inline void ncc_taps(const Hoist<NT, L>& hs, const float H[9], const 
 //    are the same numbers; on the pad the value only depends on (v00, d10)
 //    or (v00, d01) of the edge cell, whose other terms are multiplied by
 //    exact zeros.
-template <int NT, int L, int PR, bool ALLFIRST>
-__device__ inline float ncc_eval_t(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+template <int NT, int L, int PR, bool ALLFIRST, class HS>
+__device__ inline float ncc_eval_t(const HS& hs, const float H[9], const QuadSrc& src,
                                    const SrcGrid& g) {
     using B = TapBlock<NT, L>;
     constexpr int NG = (ALLFIRST && L == 4) ? B::NR : 1;
@@ -687,12 +702,12 @@ __device__ inline float ncc_eval_t(const Hoist<NT, L>& hs, const float H[9], con
     if (cx >= (float)g.w || cx < 0.0f || cy >= (float)g.h || cy < 0.0f) return 2.0f;
     // wave-uniform choice (the ballot covers the lanes still active here)
     const bool fast =
-        g.fast && __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L>(hs, H, zc, g)) == 0;
+        g.fast && __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L, HS>(hs, H, zc, g)) == 0;
     float S1 = 0.f, S2 = 0.f, X = 0.f;
     if (fast)
-        ncc_taps<NT, L, PR, NG, true>(hs, H, src, g, S1, S2, X);
+        ncc_taps<NT, L, PR, NG, true, HS>(hs, H, src, g, S1, S2, X);
     else
-        ncc_taps<NT, L, PR, NG, false>(hs, H, src, g, S1, S2, X);
+        ncc_taps<NT, L, PR, NG, false, HS>(hs, H, src, g, S1, S2, X);
     S1 = group_sum<L>(S1);
     S2 = group_sum<L>(S2);
     X = group_sum<L>(X);
@@ -703,16 +718,16 @@ __device__ inline float ncc_eval_t(const Hoist<NT, L>& hs, const float H[9], con
     return fmaxf(0.0f, fminf(2.0f, 1.0f - cov * __builtin_amdgcn_rsqf(hs.var_r * var_s)));
 }
 
-template <int NT, int L, int PR>
-__device__ inline float ncc_eval(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+template <int NT, int L, int PR, class HS>
+__device__ inline float ncc_eval(const HS& hs, const float H[9], const QuadSrc& src,
                                  const SrcGrid& g) {
-    return ncc_eval_t<NT, L, PR, false>(hs, H, src, g);
+    return ncc_eval_t<NT, L, PR, false, HS>(hs, H, src, g);
 }
 
-template <int NT, int L, int PR>
-__device__ inline float ncc_eval_mlp(const Hoist<NT, L>& hs, const float H[9], const QuadSrc& src,
+template <int NT, int L, int PR, class HS>
+__device__ inline float ncc_eval_mlp(const HS& hs, const float H[9], const QuadSrc& src,
                                      const SrcGrid& g) {
-    return ncc_eval_t<NT, L, PR, true>(hs, H, src, g);
+    return ncc_eval_t<NT, L, PR, true, HS>(hs, H, src, g);
 }
 
 }  // namespace gmvs

@@ -1136,13 +1136,10 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
         const int sl = (int)(code & 63u), v = (int)(code >> 6);
         const Active a = active_slot(P, tile, color, sl);
         const float fpx = (float)a.px, fpy = (float)a.py;
-        Hoist<NT, L> hs;
-#pragma unroll
-        for (int i = 0; i < NW; ++i) {
-            hs.wt[i] = S.hw[sl][sub][i];
-            hs.wr[i] = S.hw[sl][sub][NW + i];
-        }
-        hoist_taps<NT, L>(hs, a.px, a.py, P.h, P.w, P.radius, P.inc, sub);
+        HoistRef<NT, L> hs;
+        hs.wt = &S.hw[sl][sub][0];
+        hs.wr = &S.hw[sl][sub][NW];
+        hoist_taps<NT, L, HoistRef<NT, L>>(hs, a.px, a.py, P.h, P.w, P.radius, P.inc, sub);
         hs.inv_w = S.hp[sl][0];
         hs.mean_r = S.hp[sl][1];
         hs.var_r = S.hp[sl][2];

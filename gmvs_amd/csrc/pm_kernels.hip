@@ -652,7 +652,12 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
         // out that iteration): LDS and cost-table offsets stay scalar
 #pragma unroll 1
         for (int k = 0; k < 9; ++k) {
-            if (!((todo >> k) & 1u)) continue;
+            if (!((todo >> k) & 1u)) {
+                // an unflagged candidate keeps the reference's cost_array
+                // default (lib 854, src 717): every entry but [0][0] is zero
+                if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = (k == 0 && v == 0) ? 2.0f : 0.0f;
+                continue;
+            }
             const float m[3] = {s_m[k][0][threadIdx.x], s_m[k][1][threadIdx.x], s_m[k][2][threadIdx.x]};
             float H[9];
             homography(ab, ab + 9, m, H);
@@ -753,12 +758,14 @@ __device__ inline float prior_score(const PMConst& P, float cost, float depth, f
     return fexp(-cost * cost / 0.18f) * pr;
 }
 
-__device__ inline float ca_entry(const float* __restrict__ ca, uint32_t flags, int k, int v, int nv,
-                                 int NA, int aidx) {
-    // cost_array[8][32] = {2.0f} leaves every entry but [0][0] at zero
-    // (lib 854, src 717); unflagged rows keep those values.
-    if ((flags >> k) & 1u) return ca[((size_t)k * nv + v) * NA + aidx];
-    return (k == 0 && v == 0) ? 2.0f : 0.0f;
+// Cost of candidate k against view v. pm_candidates writes every row: the
+// NCC of a flagged candidate, and for an unflagged one the reference's
+// cost_array default (cost_array[8][32] = {2.0f} leaves every entry but
+// [0][0] at zero, lib 854, src 717), so the read needs no branch and several
+// can be in flight.
+__device__ inline float ca_entry(const float* __restrict__ ca, int k, int v, int nv, int NA,
+                                 int aidx) {
+    return ca[((size_t)k * nv + v) * NA + aidx];
 }
 
 // Per-pixel record handed from pm_select to pm_refine (workspace, [word][NA]).
@@ -848,13 +855,18 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
                 if ((flags >> (2 * i)) & 1u) pri += ((nbsel[i] >> v) & 1u) ? 0.9f : 0.1f;
             float count = 0.f, tmpw = 0.f;
             int cf = 0;
+            float xs[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) xs[k] = ca_entry(ca, k, v, nv, NA, aidx);
+            // branch-free (adding +0 leaves the sums' bits unchanged), so the
+            // loads above are all issued before the first use
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
-                if (x < tau) {
-                    tmpw += fexp(x * x / -0.18f);
-                    count += 1.0f;
-                }
+                const float x = xs[k];
+                const bool in = x < tau;
+                const float e = fexp(x * x / -0.18f);
+                tmpw += in ? e : 0.0f;
+                count += in ? 1.0f : 0.0f;
                 cf += (x > 1.2f);
             }
             float p = 0.f;
@@ -922,7 +934,7 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
             for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
                 const int v = __ffs(mm) - 1;
                 const float wv = nib_weight(wnib, v);
-                const float x = ca_entry(ca, flags, k, v, nv, NA, aidx);
+                const float x = ca_entry(ca, k, v, nv, NA, aidx);
                 if (P.geom && fk)
                     s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cp, fpx, fpy));
                 else

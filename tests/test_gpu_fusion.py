@@ -63,3 +63,52 @@ def test_fusion_bit_exact_against_twin(thresholds):
         total_o += int((od_o > 0).sum())
     print(f"fused points: GPU {total_g}  twin {total_o}")
     assert total_g == total_o and total_g > 0
+
+
+def test_fusion_bit_exact_full_size():
+    """The fusion configuration's view size and thresholds (1600x1200, >= 3
+    views, 1 px, 0.01, 30 deg), bit-exact against the twin on every view; 8
+    views with 7 sources each keep the sequential CPU twin within seconds."""
+    from gmvs_amd import src as S
+    from gmvs_amd.synthetic import make_scene
+
+    dev = torch.device("cuda")
+    sc = make_scene(1600, 1200, 8, 64, seed=3407 + 5, fx=1500.0, arc_deg=30.0, device=dev)
+    depth, normal = sc.depth.cpu(), sc.normal.cpu()
+    rng = np.random.default_rng(11)
+    ds, ns = [], []
+    for i in range(8):
+        d = depth[i].numpy().copy()
+        d *= (1 + 0.01 * rng.standard_normal(d.shape)).astype(np.float32)
+        out = rng.random(d.shape) < 0.05
+        d[out] = rng.uniform(1.5, 7.0, out.sum()).astype(np.float32)
+        ds.append(d.astype(np.float32))
+        ns.append(normal[i].numpy().astype(np.float32).copy())
+    cams_c = [S.load_camera(c.K.reshape(-1), c.R.reshape(-1), c.t, c.height, c.width,
+                            c.depth_min, c.depth_max)._c for c in sc.cameras]
+    ocams = T.oracle_cams(sc, range(8))
+    dd = [torch.from_numpy(d).to(dev) for d in ds]
+    nd = [torch.from_numpy(n).to(dev) for n in ns]
+    h, w = ds[0].shape
+    masks_g = torch.zeros(8, h, w, dtype=torch.bool, device=dev)
+    masks_o = np.zeros((8, h, w), np.uint8)
+    ang = math.radians(30.0)
+    cos_t = float(np.float32(math.cos(ang)))
+    total = 0
+    for ref in range(8):
+        ids = [ref] + list(sc.pairs[ref][:7])
+        od = torch.empty(h, w, device=dev)
+        on = torch.empty(h, w, 3, device=dev)
+        S.fuse_view(ids, [cams_c[i] for i in ids], [dd[i] for i in ids], [nd[i] for i in ids],
+                    masks_g, 3, od, on, 1.0, 0.01, ang)
+        torch.cuda.synchronize()
+        od_o, on_o, _ = O.fuse_view([ocams[i] for i in ids], [ds[i] for i in ids],
+                                    [ns[i] for i in ids], masks_o, ref, ids[1:], 3, 1.0, 0.01,
+                                    cos_t)
+        gd, gn = od.cpu().numpy(), on.cpu().numpy()
+        assert np.array_equal(gd.view(np.uint32), od_o.view(np.uint32)), f"depth of view {ref}"
+        assert np.array_equal(gn.view(np.uint32), on_o.view(np.uint32)), f"normal of view {ref}"
+        assert np.array_equal(masks_g.cpu().numpy().astype(np.uint8), masks_o), f"masks after {ref}"
+        total += int((gd > 0).sum())
+    print(f"fused points (8 views 1600x1200): {total}")
+    assert total > 0

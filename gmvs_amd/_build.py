@@ -5,7 +5,8 @@ instructions are disabled (-packed-fp32-ops). FP32 division/sqrt are
 correctly rounded except in pm_kernels.hip, whose results are checked against
 the FP64 oracle within a tolerance. fusion.hip additionally disables FMA
 contraction so that its arithmetic matches a sequential CPU loop bit for bit;
-so does post.hip (point gather, text export, segment utilities).
+so do post.hip (point gather, text export, segment utilities) and
+prior_gpu.hip (device planar-prior construction, equal to the host one).
 """
 from __future__ import annotations
 
@@ -44,6 +45,7 @@ SOURCES = {
     "pm_kernels.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "fusion.hip": ["-ffp-contract=off"],
     "post.hip": ["-ffp-contract=off"],
+    "prior_gpu.hip": ["-ffp-contract=off"],
 }
 
 

@@ -120,6 +120,8 @@ EXPORTS = (
     "gmvs_fuse_workspace_bytes",
     "gmvs_fuse_view",
     "gmvs_prior_build",
+    "gmvs_prior_device_workspace_bytes",
+    "gmvs_prior_build_device",
     "gmvs_delaunay",
     "gmvs_points_workspace_bytes",
     "gmvs_points_gather",
@@ -201,6 +203,11 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_fuse_view.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
                                    ctypes.POINTER(vp), vp, i32, i32, vp, i32, f32, f32, f32, vp,
                                    vp, vp, sz, vp]
+    lib.gmvs_prior_device_workspace_bytes.argtypes = [i32, i32]
+    lib.gmvs_prior_device_workspace_bytes.restype = sz
+    lib.gmvs_prior_build_device.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,
+                                            ctypes.POINTER(CParams), vp, vp,
+                                            ctypes.POINTER(ctypes.c_int32), vp, sz, vp]
     lib.gmvs_prior_build.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,
                                      ctypes.POINTER(CParams), vp, vp, ctypes.POINTER(i32)]
     lib.gmvs_delaunay.argtypes = [vp, i32, vp, i32, ctypes.POINTER(i32)]

@@ -369,6 +369,17 @@ void walk_triangle(const IPt& a, const IPt& b, const IPt& c, F&& visit) {
 }
 
 }  // namespace
+
+// Delaunay triangles (counter-clockwise vertex index triples) of distinct
+// integer points in [0, 65536)^2, for the device prior construction.
+std::vector<int> host_delaunay(const int32_t* xy, int n) {
+    std::vector<Pt> pts(n);
+    for (int i = 0; i < n; ++i) pts[i] = Pt{xy[2 * i], xy[2 * i + 1]};
+    Delaunay d(pts);
+    d.run();
+    return d.triangles();
+}
+
 }  // namespace gmvs
 
 using namespace gmvs;

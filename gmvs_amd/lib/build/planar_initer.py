@@ -24,7 +24,8 @@ from ...src import (Camera, PatchMatchParams, Problem, _Workspace, _device, load
                     load_problem, run_pm)
 
 __all__ = ["PlanarInit", "PatchMatchParams", "Problem", "Camera", "load_camera", "load_problem",
-           "fusion_planar_mask", "filter_by_var_map", "sliding_window_normal", "build_prior"]
+           "fusion_planar_mask", "filter_by_var_map", "sliding_window_normal", "build_prior",
+           "build_prior_device"]
 
 
 def build_prior(ref_cam: N.CCamera, depth: np.ndarray, cost: np.ndarray,
@@ -51,6 +52,38 @@ def build_prior(ref_cam: N.CCamera, depth: np.ndarray, cost: np.ndarray,
         lab.ctypes.data if lab is not None else None, ctypes.byref(cp), planes.ctypes.data,
         masks.ctypes.data, ctypes.byref(ntri)))
     return planes, masks.astype(np.int32), int(ntri.value)
+
+
+def build_prior_device(ref_cam: N.CCamera, depth, cost, labels, depth_min: float,
+                       depth_max: float, superpixel_filter: bool = False):
+    """Device planar-prior construction (gmvs_prior_build_device; equal to
+    build_prior bit for bit). depth, cost: [H,W] float32 device tensors;
+    labels: [H,W] int32 device tensor or None. Returns (prior_planes [H,W,4]
+    f32, plane_masks [H,W] int32, #triangles), device tensors; synchronises
+    the current stream (the triangulation runs on the host)."""
+    import torch
+
+    h, w = ref_cam.height, ref_cam.width
+    dev = depth.device
+    depth = depth.to(torch.float32).contiguous().reshape(h, w)
+    cost = cost.to(dev, torch.float32).contiguous().reshape(h, w)
+    lab = None if labels is None else labels.to(dev, torch.int32).contiguous().reshape(h, w)
+    p = PatchMatchParams()
+    p.depth_min = depth_min
+    p.depth_max = depth_max
+    p.superpixel_filter = superpixel_filter
+    cp = p.to_c()
+    planes = torch.empty(h, w, 4, dtype=torch.float32, device=dev)
+    masks = torch.empty(h, w, dtype=torch.int32, device=dev)
+    nbytes = N.lib.gmvs_prior_device_workspace_bytes(h, w)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    ntri = ctypes.c_int32(0)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    N.check("gmvs_prior_build_device", N.lib.gmvs_prior_build_device(
+        ctypes.byref(ref_cam), depth.data_ptr(), cost.data_ptr(),
+        lab.data_ptr() if lab is not None else None, ctypes.byref(cp), planes.data_ptr(),
+        masks.data_ptr(), ctypes.byref(ntri), ws.data_ptr(), nbytes, stream))
+    return planes, masks, int(ntri.value)
 
 
 def delaunay(xy: np.ndarray) -> np.ndarray:
@@ -81,7 +114,7 @@ class PlanarInit:
         self._planes: Optional[torch.Tensor] = None
         self._costs: Optional[torch.Tensor] = None
         self._prior: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
-        self._host_prior: Optional[Tuple[np.ndarray, np.ndarray]] = None
+        self._built_prior: Optional[Tuple[torch.Tensor, torch.Tensor]] = None  # built prior (device)
         self._ws = _Workspace()
         # Philox key word 1 (the global reference view id; set by the driver)
         self.rng_view = 0
@@ -121,11 +154,10 @@ class PlanarInit:
         self._costs = torch.zeros(h, w, dtype=torch.float32, device=dev)
 
     def cuda_planar_prior_init(self) -> None:
-        if self._host_prior is None:
+        if self._built_prior is None:
             raise RuntimeError("triangulation must run before cuda_planar_prior_init")
-        dev = self._planes.device
-        self._prior = (torch.from_numpy(self._host_prior[0]).to(dev),
-                       torch.from_numpy(self._host_prior[1]).to(dev))
+        # the prior was built on the device by triangulation (already resident)
+        self._prior = self._built_prior
 
     # ---- PatchMatch (lib 1711-1810)
     def run_patch_match(self) -> None:
@@ -169,17 +201,25 @@ class PlanarInit:
         """planar_initer.cu:347-420: support points from the last pass's
         costs, Delaunay, superpixel filter, plane fit and rasterisation."""
         cam = self._cameras[0]
-        cost = self._costs.cpu().numpy()
-        planes, masks, ntri = build_prior(cam._c, depth, cost, self._superpixel,
-                                          self.params.depth_min, self.params.depth_max,
-                                          bool(self.params.superpixel_filter)
-                                          and self._superpixel is not None)
-        self._host_prior = (planes, masks)
+        dev = self._costs.device
+        h, w = cam.height, cam.width
+        d = torch.from_numpy(np.ascontiguousarray(depth, np.float32).reshape(h, w)).to(dev)
+        lab = None
+        if self._superpixel is not None:
+            lab = torch.from_numpy(np.ascontiguousarray(self._superpixel, np.int32)).to(dev)
+        # on the device (gmvs_prior_build_device, equal to the host build bit
+        # for bit): the cost map stays resident, only the support points and
+        # triangles cross to the host for the Delaunay step
+        planes, masks, ntri = build_prior_device(cam._c, d, self._costs, lab,
+                                                 self.params.depth_min, self.params.depth_max,
+                                                 bool(self.params.superpixel_filter)
+                                                 and self._superpixel is not None)
+        self._built_prior = (planes, masks)
         self.num_triangles = ntri
         if triangulation_path.endswith(".png"):
             if self.verbose:
                 print(f"save triangulation as {triangulation_path}")
-            self._save_prior_png(triangulation_path, masks)
+            self._save_prior_png(triangulation_path, masks.cpu().numpy())
 
     def _save_prior_png(self, path: str, masks: np.ndarray) -> None:
         img = self._images[0]

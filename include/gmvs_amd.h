@@ -217,6 +217,21 @@ int gmvs_prior_build(const gmvs_camera* ref_camera, const float* depth, const fl
                      const int32_t* labels, const gmvs_params* params, float* prior_planes,
                      uint32_t* plane_masks, int32_t* out_num_triangles);
 
+/* gmvs_prior_build on the device, with the same results bit for bit:
+ * support points, the superpixel filter, plane fit, rasterisation and the
+ * depth-range check run as kernels; the Delaunay triangulation between them
+ * runs on the host (the kept support points are copied back, the triangles
+ * up), so the call synchronises the stream. Replaces the same reference
+ * steps as gmvs_prior_build (gmvs/lib/src/planar_initer.cu:195-420,686-745).
+ * depth, cost: device [H*W] f32; labels: device [H*W] int32 or NULL;
+ * prior_planes: device [H*W*4]; plane_masks: device [H*W];
+ * workspace: device, gmvs_prior_device_workspace_bytes(H, W) bytes. */
+size_t gmvs_prior_device_workspace_bytes(int32_t height, int32_t width);
+int gmvs_prior_build_device(const gmvs_camera* ref_camera, const float* depth, const float* cost,
+                            const int32_t* labels, const gmvs_params* params, float* prior_planes,
+                            uint32_t* plane_masks, int32_t* out_num_triangles, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
 /* Delaunay triangulation used by gmvs_prior_build, exported for its tests;
  * replaces the cv::Subdiv2D insert / getTriangleList calls of
  * PlanarInit::triangulation (gmvs/lib/src/planar_initer.cu:354-361).

@@ -177,3 +177,123 @@ def test_filter_median_matches_numpy_on_constant_cost():
         if not np.all(np.isfinite(vals)):
             continue
         assert filt["depth"][y, x] == pytest.approx(float(np.median(np.float32(vals))), rel=1e-6)
+
+
+def _bilinear_pair(h, w, f_r, f_s, c_r, c_s):
+    """Reference image f(x, y) on the integer grid and a source image g with
+    g(H(x, y)) = f(x, y) for the homography H = Ks Kr^-1 (R = I, t = 0): a
+    scaling by s = f_s / f_r about the principal points. f is a saddle plus
+    a ramp, so g is bilinear in (u, v) and software bilinear interpolation of
+    g's grid reproduces it exactly at every tap."""
+    def f(x, y):
+        return 120.0 + 0.6 * (x - 40.0) * (y - 30.0) + 0.3 * x
+    s = f_s / f_r
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    ref = f(x, y)
+    src = f((x - c_s[0]) / s + c_r[0], (y - c_s[1]) / s + c_r[1])
+    return ref.astype(np.float32), src.astype(np.float32)
+
+
+def test_bilinear_sampling_exact_on_bilinear_images():
+    # A9: the taps land at fractional positions that vary over the patch
+    # (scale 1.37); any error in the cell blend (swapped fx/fy, a dropped
+    # cross term, nearest sampling) shows up as a nonzero cost.
+    h, w = 60, 80
+    f_r, f_s = 50.0, 50.0 * 1.37
+    c_r, c_s = (40.0, 30.0), (36.7, 27.9)
+    ref, src = _bilinear_pair(h, w, f_r, f_s, c_r, c_s)
+    K_r = np.array([[f_r, 0, c_r[0]], [0, f_r, c_r[1]], [0, 0, 1.0]])
+    K_s = np.array([[f_s, 0, c_s[0]], [0, f_s, c_s[1]], [0, 0, 1.0]])
+    cams = [_cam(K_r, np.eye(3), [0, 0, 0], h, w), _cam(K_s, np.eye(3), [0, 0, 0], h, w)]
+    rng = np.random.default_rng(5)
+    xy = np.stack([rng.integers(25, 56, 40), rng.integers(18, 43, 40)], 1).astype(np.int32)
+    planes = np.stack([_fronto(3.0)] * len(xy))
+    c, _, _ = O.cost_batch(cams, [ref, src], O.make_params(num_images=2), xy, planes)
+    assert np.all(c < 1e-6), c.max()
+    # the same images one pixel off in the source's x direction do not match
+    # (the saddle is not shift-invariant)
+    src_off = np.roll(src, 1, axis=1)
+    c_off, _, _ = O.cost_batch(cams, [ref, src_off], O.make_params(num_images=2), xy, planes)
+    assert np.median(c_off) > 1e-4
+
+
+def _init_only(sc, ids, **kw):
+    dmin, dmax = T.depth_range(sc, 0)
+    p = O.make_params(max_iterations=0, depth_min=dmin, depth_max=dmax, num_images=len(ids), **kw)
+    return O.pm_run(T.oracle_cams(sc, ids), [sc.images[i].numpy() for i in ids], p, precision=64,
+                    rng_view=0, skip_filter=True), (dmin, dmax)
+
+
+def _camera_frame(sc, ref, depth, normal_w):
+    cam = sc.cameras[ref]
+    K = np.asarray(cam.K, np.float64)
+    R = np.asarray(cam.R, np.float64)
+    H, W = depth.shape
+    y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+    ray = np.stack([(x - K[0, 2]) / K[0, 0], (y - K[1, 2]) / K[1, 1], np.ones_like(x)], -1)
+    n_c = normal_w.astype(np.float64) @ R.T  # n_w = R^T n_c
+    X = ray * depth.astype(np.float64)[..., None]
+    return ray, n_c, X
+
+
+def test_random_init_normals_uniform_on_the_facing_hemisphere():
+    # A16: Marsaglia's method gives a uniform direction on the sphere; flipped
+    # to face the camera it is uniform on the hemisphere, so cos(angle to the
+    # reversed viewing ray) is uniform on [0, 1] (Archimedes). The depth draw
+    # is uniform on [dmin, dmax].
+    sc = T.scene()
+    o, (dmin, dmax) = _init_only(sc, [0, 1, 2])
+    ray, n_c, _ = _camera_frame(sc, 0, o["depth"], o["normal"])
+    norm = np.linalg.norm(n_c, axis=-1)
+    assert np.abs(norm - 1.0).max() < 1e-6
+    cosv = -(n_c * ray).sum(-1) / np.linalg.norm(ray, axis=-1)
+    assert cosv.min() >= -1e-6
+    n = cosv.size
+    hist, _ = np.histogram(np.clip(cosv, 0, 1), bins=10, range=(0, 1))
+    chi2 = float(((hist - n / 10) ** 2 / (n / 10)).sum())
+    assert chi2 < 27.9, chi2  # 9 dof, p = 0.001
+    # the azimuth about the viewing ray is uniform too: the sign of the
+    # normal's x component in the camera frame splits the draws evenly
+    assert abs(float((n_c[..., 0] > 0).mean()) - 0.5) < 0.01
+    d = o["depth"].ravel().astype(np.float64)
+    assert d.min() >= dmin * (1 - 1e-6) and d.max() <= dmax * (1 + 1e-6)
+    hist, _ = np.histogram(d, bins=10, range=(dmin, dmax))
+    chi2 = float(((hist - d.size / 10) ** 2 / (d.size / 10)).sum())
+    assert chi2 < 27.9, chi2
+
+
+def test_random_init_cost_is_mean_of_top_k_views():
+    # A13: the initial cost is the mean of the k = min(top_k, #valid) smallest
+    # view costs of the init plane, and the selected views are those at or
+    # below the k-th; checked against numpy's sort of the per-view costs.
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    o, _ = _init_only(sc, ids, top_k=3)
+    ray, n_c, X = _camera_frame(sc, 0, o["depth"], o["normal"])
+    rng = np.random.default_rng(11)
+    H, W = o["depth"].shape
+    ys, xs = rng.integers(0, H, 400), rng.integers(0, W, 400)
+    planes = np.concatenate([n_c[ys, xs], -(n_c[ys, xs] * X[ys, xs]).sum(-1, keepdims=True)], 1)
+    xy = np.stack([xs, ys], 1).astype(np.int32)
+    c, _, _ = O.cost_batch(T.oracle_cams(sc, ids), [sc.images[i].numpy() for i in ids],
+                           O.make_params(num_images=5), xy, planes.astype(np.float32))
+    # (the plane is rebuilt from the FP32-rounded depth and normal outputs, so
+    # costs at steeply slanted planes move by up to ~1e-3)
+    err, checked = [], 0
+    for i in range(len(xy)):
+        v = c[i].astype(np.float64)
+        k = min(3, int((v < 2.0).sum()))
+        cost = o["cost"][ys[i], xs[i]]
+        if k == 0:
+            assert cost == 2.0
+            continue
+        srt = np.sort(v)
+        err.append(abs(cost - srt[:k].mean()))
+        # the selection bits, where the k-th cost is not a near tie
+        if k < len(srt) and srt[k] - srt[k - 1] > 1e-2:
+            bits = sum(1 << j for j in range(len(v)) if v[j] <= srt[k - 1])
+            assert int(o["selected_views"][ys[i], xs[i]]) == bits
+            checked += 1
+    err = np.asarray(err)
+    assert np.quantile(err, 0.95) < 1e-5 and err.max() < 5e-3, (np.quantile(err, 0.95), err.max())
+    assert checked > 300

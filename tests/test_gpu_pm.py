@@ -234,3 +234,28 @@ def test_pm_init_only_matches_oracle(mode):
     print(f"init {mode}: agreement vs FP64 oracle = {agree:.5f}; cost |err| p99 = "
           f"{np.quantile(cerr, 0.99):.2e}")
     assert agree >= 0.999
+
+
+def test_cost_batch_exact_on_bilinear_images():
+    """Software bilinear on the device reproduces a bilinear source image at
+    every tap (a saddle plus a ramp, warped by a 1.37x scaling): the cost is
+    zero up to FP32 rounding, as the oracle's is (test_cpu_oracle.py)."""
+    from types import SimpleNamespace
+
+    from tests.test_cpu_oracle import _bilinear_pair
+
+    h, w = 60, 80
+    f_r, f_s = 50.0, 50.0 * 1.37
+    c_r, c_s = (40.0, 30.0), (36.7, 27.9)
+    ref, src = _bilinear_pair(h, w, f_r, f_s, c_r, c_s)
+    cams = [SimpleNamespace(K=np.array([[f, 0, c[0]], [0, f, c[1]], [0, 0, 1.0]], np.float32),
+                            R=np.eye(3, dtype=np.float32), t=np.zeros(3, np.float32), height=h,
+                            width=w, depth_min=1.0, depth_max=10.0)
+            for f, c in ((f_r, c_r), (f_s, c_s))]
+    sc = SimpleNamespace(images=[torch.from_numpy(ref), torch.from_numpy(src)], cameras=cams)
+    rng = np.random.default_rng(5)
+    xy = np.stack([rng.integers(25, 56, 40), rng.integers(18, 43, 40)], 1).astype(np.int32)
+    planes = np.stack([np.array([0.0, 0.0, -1.0, 3.0], np.float32)] * len(xy))
+    gpu = gpu_cost_batch(sc, [0, 1], xy, planes)
+    print(f"bilinear images: max cost {gpu.max():.2e}")
+    assert np.all(gpu < 1e-5), gpu.max()

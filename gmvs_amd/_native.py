@@ -30,6 +30,12 @@ PREC_FP16 = 1
 PREC_BF16 = 2
 PRECISIONS = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16}
 
+# gmvs_pm_args.compat bits (GMVS_COMPAT_*): 0 reproduces the reference's quirks
+COMPAT_FIX_RIGHT_FAR = 1
+COMPAT_FIX_COST_DEFAULT = 2
+COMPAT_FIX_PRIOR_NORMALS = 4
+COMPAT_FIX_PRIOR_SELECTION = 8
+
 TUNE_INIT = 0
 TUNE_CANDIDATES = 1
 TUNE_REFINE = 2
@@ -99,6 +105,7 @@ class CPMArgs(ctypes.Structure):
         ("out_normal", ctypes.c_void_p),
         ("out_cost", ctypes.c_void_p),
         ("tile_cols", ctypes.c_int32),
+        ("compat", ctypes.c_uint32),
     ]
 
 

@@ -86,6 +86,7 @@ struct PMConst {
     int top_k;
     uint32_t seed, view;
     int init_mode, geom, prior;
+    uint32_t compat;              // GMVS_COMPAT_* (0: the reference's quirks)
     TileShape tile;
     __device__ SrcGrid grid() const { return SrcGrid{h, w, wp, fast_taps}; }
 };
@@ -317,7 +318,13 @@ __device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restri
         // backend (the reuse path read registers clobbered by the mask load).
         const float4 st = planes[c];
         pl = st;
-        pl.w = dist_to_origin(rc, fpx, fpy, st.w, st);
+        if (P.compat & GMVS_COMPAT_FIX_PRIOR_NORMALS) {
+            // the first pass exported world-frame normals (GetDepthandNormal)
+            pl.x = rc.R[0] * st.x + rc.R[1] * st.y + rc.R[2] * st.z;
+            pl.y = rc.R[3] * st.x + rc.R[4] * st.y + rc.R[5] * st.z;
+            pl.z = rc.R[6] * st.x + rc.R[7] * st.y + rc.R[8] * st.z;
+        }
+        pl.w = dist_to_origin(rc, fpx, fpy, st.w, pl);
         bool perturb = false;
         if (P.prior) {
             const uint32_t mval = P.plane_masks[c];
@@ -438,7 +445,7 @@ __device__ inline Active active_pixel(const PMConst& P, int tile, int color) {
 // src 720-913). Order: 0 up_near, 1 up_far, 2 down_near, 3 down_far,
 // 4 left_near, 5 left_far, 6 right_near, 7 right_far.
 __device__ inline void candidate_search(const float* __restrict__ cs, int px, int py, int h, int w,
-                                        uint32_t& flags, int pos[8]) {
+                                        bool right_far_min, uint32_t& flags, int pos[8]) {
     const int c = py * w + px;
     flags = 0;
 #pragma unroll
@@ -496,7 +503,9 @@ __device__ inline void candidate_search(const float* __restrict__ cs, int px, in
             if (px < w - 3 - 2 * i) {
                 const int q = c + 3 + 2 * i;
                 const float x = cs[q];
-                if (m < x) {  // the reference keeps the maximum here (src 794)
+                // the reference keeps the maximum here (src 794) unless
+                // GMVS_COMPAT_FIX_RIGHT_FAR
+                if (right_far_min ? (x < m) : (m < x)) {
                     m = x;
                     best = q;
                 }
@@ -617,7 +626,8 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     const bool lead = a.sub == 0;
     uint32_t flags;
     int pos[9];
-    candidate_search(costs, a.px, a.py, P.h, P.w, flags, pos);
+    candidate_search(costs, a.px, a.py, P.h, P.w, (P.compat & GMVS_COMPAT_FIX_RIGHT_FAR) != 0u, flags,
+                     pos);
     if (lead) {
         cflag[a.aidx] = (uint8_t)flags;
         ncc += (uint32_t)((__popc(flags) + 1) * P.nv);
@@ -627,6 +637,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     // hypotheses 0-7: the flagged candidates; 8: the pixel's own current
     // plane against every view (the selection weights the selected ones)
     pos[8] = a.py * P.w + a.px;
+    const bool cost_default_2 = (P.compat & GMVS_COMPAT_FIX_COST_DEFAULT) != 0u;
     const uint32_t todo = flags | (1u << 8);
     // each lane keeps the rank-1 factors m of its pixel's hypotheses in its
     // own LDS column, so the view loop reads 3 floats per NCC
@@ -655,7 +666,10 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
             if (!((todo >> k) & 1u)) {
                 // an unflagged candidate keeps the reference's cost_array
                 // default (lib 854, src 717): every entry but [0][0] is zero
-                if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = (k == 0 && v == 0) ? 2.0f : 0.0f;
+                // (2 everywhere with GMVS_COMPAT_FIX_COST_DEFAULT)
+                if (lead)
+                    ca[((size_t)k * P.nv + v) * NA + a.aidx] =
+                        (cost_default_2 || (k == 0 && v == 0)) ? 2.0f : 0.0f;
                 continue;
             }
             const float m[3] = {s_m[k][0][threadIdx.x], s_m[k][1][threadIdx.x], s_m[k][2][threadIdx.x]};
@@ -1004,7 +1018,9 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
                 depth_now = db;
                 cur = cp;
                 cost = fc[kmin];
-                if (!P.prior) selc = selmask;  // the prior path leaves it (lib 1306-1318)
+                // the prior path leaves it (lib 1306-1318) unless
+                // GMVS_COMPAT_FIX_PRIOR_SELECTION
+                if (!P.prior || (P.compat & GMVS_COMPAT_FIX_PRIOR_SELECTION)) selc = selmask;
             }
         }
 
@@ -1612,6 +1628,7 @@ int check_args(const gmvs_pm_args& A, int* radius) {
     GMVS_ARG_CHECK(A.tile_cols == 0 || A.tile_cols == 4 || A.tile_cols == 8 || A.tile_cols == 16 ||
                        A.tile_cols == 32,
                    "tile_cols must be 0, 4, 8, 16 or 32 (got %d)", A.tile_cols);
+    GMVS_ARG_CHECK((A.compat & ~15u) == 0u, "unknown compat bits 0x%x", A.compat & ~15u);
     if (A.init_mode == GMVS_INIT_DEPTH_NORMAL)
         GMVS_ARG_CHECK(A.depths && A.depths[0] && A.ref_normal,
                        "INIT_DEPTH_NORMAL needs depths[0] and ref_normal");
@@ -1719,6 +1736,7 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
         P.init_mode = A.init_mode;
         P.geom = A.geom ? 1 : 0;
         P.prior = A.params.planar_prior ? 1 : 0;
+        P.compat = A.compat;
         P.tile.shift = tile_shift(A.tile_cols);
         pr.planes = reinterpret_cast<float4*>(A.planes);
         pr.costs = A.costs;

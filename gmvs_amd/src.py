@@ -191,9 +191,12 @@ def make_pm_args(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], para
                  out_depth: Optional[torch.Tensor] = None,
                  out_normal: Optional[torch.Tensor] = None,
                  out_cost: Optional[torch.Tensor] = None, seed: int = RANDOM_SEED,
-                 skip_filter: bool = False, precision: int = N.PREC_FP32, tile_cols: int = 0):
+                 skip_filter: bool = False, precision: int = N.PREC_FP32, tile_cols: int = 0,
+                 compat: int = 0):
     """One gmvs_pm_args record over device tensors. Returns (args, keep): keep
-    holds the host arrays the record points to and must outlive the call."""
+    holds the host arrays the record points to and must outlive the call.
+    compat: N.COMPAT_* bits switching reference quirks to their corrected
+    behaviour (0: the reference's)."""
     n = len(images)
     cam_arr = N.camera_array(cams)
     img_arr = N.ptr_array([im.data_ptr() for im in images])
@@ -219,6 +222,7 @@ def make_pm_args(cams: Sequence[N.CCamera], images: Sequence[torch.Tensor], para
     a.out_normal = out_normal.data_ptr() if out_normal is not None else None
     a.out_cost = out_cost.data_ptr() if out_cost is not None else None
     a.tile_cols = tile_cols
+    a.compat = compat
     return a, (cam_arr, img_arr, dep_arr)
 
 

@@ -85,8 +85,31 @@ enum {
  * NCC moments are accumulated in FP32 about the patch-centre intensity. */
 enum {
     GMVS_PREC_FP32 = 0, /* plain FP32 path: 16-byte bilinear cells, FP32 FMAs */
-    GMVS_PREC_FP16 = 1, /* 8-byte FP16 cells, bilinear as FP16 dot products, FP32 sums */
+    GMVS_PREC_FP16 = 1, /* 8-byte FP16 cells, bilinear as FP32 FMAs of the widened
+                           FP16 operands (exact for 8-bit images), FP32 sums */
     GMVS_PREC_BF16 = 2  /* 8-byte BF16 cells, bilinear as BF16 dot products, FP32 sums */
+};
+
+/* Reference quirks (gmvs_pm_args.compat). 0 mirrors the reference in each
+ * case; a set bit switches that one quirk to the corrected behaviour, so
+ * both can be run and compared. The oracle implements the same bits. */
+enum {
+    /* right_far keeps the minimum cost; the reference keeps the maximum
+       (`costMin < costs[...]`, lib 954, src 794) */
+    GMVS_COMPAT_FIX_RIGHT_FAR = 1,
+    /* an out-of-image (unflagged) candidate costs 2 against every view; the
+       reference's cost_array[8][32] = {2.0f} sets only [0][0] and leaves the
+       rest 0 (lib 854, src 717), so such candidates look good to the view
+       selection */
+    GMVS_COMPAT_FIX_COST_DEFAULT = 2,
+    /* GMVS_INIT_PRIOR rotates the reused first-pass normals, which are in
+       the world frame, into the camera frame; the reference uses them as
+       camera-frame normals (lib 1546-1550 after 1672-1673) */
+    GMVS_COMPAT_FIX_PRIOR_NORMALS = 4,
+    /* the planar-prior pass updates the selected views when an unmasked
+       pixel adopts a candidate, as the plain pass does; the reference leaves
+       them (lib 1306-1318 against 1332) */
+    GMVS_COMPAT_FIX_PRIOR_SELECTION = 8
 };
 
 /* One PatchMatch pass over one reference view. */
@@ -115,6 +138,7 @@ typedef struct gmvs_pm_args {
     int32_t tile_cols;            /* shape of the 64-pixel work tile: columns 4, 8, 16 or 32
                                      (rows = 64 / cols); 0 = 8 x 8. Results do not depend
                                      on it (chosen by gmvs_amd.autotune) */
+    uint32_t compat;              /* GMVS_COMPAT_* bits; 0 = the reference's behaviour */
 } gmvs_pm_args;
 
 /* Last error message of the calling thread ("" if none). */

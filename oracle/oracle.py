@@ -63,7 +63,7 @@ class OPMArgs(ctypes.Structure):
                 ("seed", ctypes.c_uint32), ("rng_view", ctypes.c_uint32),
                 ("threads", ctypes.c_int32), ("skip_filter", ctypes.c_int32),
                 ("out_depth", ctypes.c_void_p), ("out_normal", ctypes.c_void_p),
-                ("out_cost", ctypes.c_void_p)]
+                ("out_cost", ctypes.c_void_p), ("compat", ctypes.c_uint32)]
 
 
 def _load():
@@ -190,7 +190,8 @@ def pm_run(cams: Sequence[OCamera], images: Sequence[np.ndarray], params: OParam
            rng_view: int = 0, depths: Optional[Sequence[np.ndarray]] = None,
            ref_normal: Optional[np.ndarray] = None, prior_planes: Optional[np.ndarray] = None,
            plane_masks: Optional[np.ndarray] = None, planes: Optional[np.ndarray] = None,
-           costs: Optional[np.ndarray] = None, threads: int = 0, skip_filter: bool = False):
+           costs: Optional[np.ndarray] = None, threads: int = 0, skip_filter: bool = False,
+           compat: int = 0):
     """One PatchMatch pass. Returns dict(depth, normal, cost, planes, selected_views)."""
     imgs = [_f32(a) for a in images]
     h, w = imgs[0].shape
@@ -227,6 +228,7 @@ def pm_run(cams: Sequence[OCamera], images: Sequence[np.ndarray], params: OParam
     a.rng_view = rng_view
     a.threads = threads
     a.skip_filter = 1 if skip_filter else 0
+    a.compat = compat
     a.out_depth = _ptr(od)
     a.out_normal = _ptr(on)
     a.out_cost = _ptr(oc)

@@ -67,7 +67,16 @@ typedef struct OPMArgs {
     float* out_depth;             // [H*W] or NULL
     float* out_normal;            // [H*W*3] or NULL
     float* out_cost;              // [H*W] or NULL
+    uint32_t compat;              // GMVS_COMPAT_* bits of the product (0: the reference's quirks)
 } OPMArgs;
+
+// The product's GMVS_COMPAT_* bits (include/gmvs_amd.h), restated.
+enum : uint32_t {
+    OCOMPAT_FIX_RIGHT_FAR = 1,
+    OCOMPAT_FIX_COST_DEFAULT = 2,
+    OCOMPAT_FIX_PRIOR_NORMALS = 4,
+    OCOMPAT_FIX_PRIOR_SELECTION = 8,
+};
 
 // ---------------------------------------------------------------- Philox ---
 static inline void oracle_philox_round(uint32_t c[4], const uint32_t k[2]) {

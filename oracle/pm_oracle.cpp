@@ -476,7 +476,14 @@ struct Run {
                 pl.w = dp;
             } else {
                 pl = V4<T>{T(st[0]), T(st[1]), T(st[2]), T(st[3])};
-                pl.w = dist_to_origin(rc, px, py, pl.w, pl);
+                if (a.compat & OCOMPAT_FIX_PRIOR_NORMALS) {
+                    // world -> camera frame, as the depth/normal init does
+                    const T n0 = pl.x, n1 = pl.y, n2 = pl.z;
+                    pl.x = rc.R[0] * n0 + rc.R[1] * n1 + rc.R[2] * n2;
+                    pl.y = rc.R[3] * n0 + rc.R[4] * n1 + rc.R[5] * n2;
+                    pl.z = rc.R[6] * n0 + rc.R[7] * n1 + rc.R[8] * n2;
+                }
+                pl.w = dist_to_origin(rc, px, py, T(st[3]), pl);
             }
         } else {
             // GenerateRandomPlaneHypothesis (lib 215-230).
@@ -550,7 +557,10 @@ struct Run {
             for (int i = 1; i < 11; ++i)
                 if (px < w - 3 - 2 * i) {
                     const int q = c + 3 + 2 * i;
-                    if (m < cs[q]) {  // keeps the maximum, as the reference does
+                    // keeps the maximum, as the reference does, unless
+                    // OCOMPAT_FIX_RIGHT_FAR
+                    const bool better = (a.compat & OCOMPAT_FIX_RIGHT_FAR) ? (cs[q] < m) : (m < cs[q]);
+                    if (better) {
                         m = cs[q];
                         best = q;
                     }
@@ -684,8 +694,9 @@ struct Run {
         int pos[8];
         candidates(px, py, flag, pos);
 
-        // cost_array[8][32] = {2.0f}: only [0][0] is 2, the rest are 0.
-        std::vector<T> ca(size_t(8) * nv, T(0));
+        // cost_array[8][32] = {2.0f}: only [0][0] is 2, the rest are 0
+        // (all 2 with OCOMPAT_FIX_COST_DEFAULT).
+        std::vector<T> ca(size_t(8) * nv, (a.compat & OCOMPAT_FIX_COST_DEFAULT) ? T(2) : T(0));
         ca[0] = T(2);
         for (int k = 0; k < 8; ++k)
             if (flag[k])
@@ -807,6 +818,7 @@ struct Run {
                     depth_now = db;
                     plane[c] = plane0[pos[kmin]];
                     cost[c] = fc[kmin];
+                    if (a.compat & OCOMPAT_FIX_PRIOR_SELECTION) sel[c] = tsel;
                 }
             }
         } else if (flag[kmin]) {

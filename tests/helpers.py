@@ -89,3 +89,23 @@ def depth_normal_agreement(d_a, n_a, d_b, n_b, rel_tol=0.005, ang_tol=1.0):
     ok = valid & (rel <= rel_tol) & (ang <= ang_tol)
     both_nan = ~np.isfinite(d_a) & ~np.isfinite(d_b)
     return float((ok | both_nan).mean())
+
+
+def world_state_facing_camera(sc, ref, seed=0):
+    """A [H, W, 4] (world normal, depth) state as the first pass exports it:
+    the ground-truth depth and random unit normals within ~30 degrees of the
+    reversed viewing ray (well conditioned for the depth <-> plane round
+    trip), rotated into the world frame (n_w = R^T n_c)."""
+    cam = sc.cameras[ref]
+    K = np.asarray(cam.K, np.float64)
+    R = np.asarray(cam.R, np.float64)
+    H, W = cam.height, cam.width
+    y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+    ray = np.stack([(x - K[0, 2]) / K[0, 0], (y - K[1, 2]) / K[1, 1], np.ones_like(x)], -1)
+    ray /= np.linalg.norm(ray, axis=-1, keepdims=True)
+    rng = np.random.default_rng(seed)
+    n_c = -ray + 0.3 * rng.standard_normal((H, W, 3))
+    n_c /= np.linalg.norm(n_c, axis=-1, keepdims=True)
+    n_w = n_c @ R  # row form of R^T n_c
+    depth = sc.depth[ref].numpy().astype(np.float64)
+    return np.concatenate([n_w, depth[..., None]], -1).astype(np.float32)

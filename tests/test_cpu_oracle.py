@@ -297,3 +297,46 @@ def test_random_init_cost_is_mean_of_top_k_views():
     err = np.asarray(err)
     assert np.quantile(err, 0.95) < 1e-5 and err.max() < 5e-3, (np.quantile(err, 0.95), err.max())
     assert checked > 300
+
+
+def test_compat_prior_normals_round_trip():
+    # Q5 switched (OCOMPAT_FIX_PRIOR_NORMALS): an init_mode 2 init without a
+    # prior reads the world-frame state, rotates it into the camera frame,
+    # and GetDepthandNormal returns it unchanged; mirrored, the normals come
+    # back rotated by R^T (the reference's reuse of world normals).
+    sc = T.scene()
+    ids = [0, 1, 2]
+    st = T.world_state_facing_camera(sc, 0, seed=4)
+    h, w = st.shape[:2]
+    dmin, dmax = T.depth_range(sc, 0)
+    p = O.make_params(max_iterations=0, depth_min=dmin, depth_max=dmax, num_images=3)
+    cams = T.oracle_cams(sc, ids)
+    imgs = [sc.images[i].numpy() for i in ids]
+    kw = dict(precision=64, init_mode=2, planes=st, costs=np.ones((h, w), np.float32),
+              rng_view=0, skip_filter=True)
+    o = O.pm_run(cams, imgs, p, compat=4, **kw)
+    assert np.abs(o["normal"] - st[..., :3]).max() < 1e-6
+    assert np.abs(o["depth"] / st[..., 3] - 1).max() < 1e-6
+    o0 = O.pm_run(cams, imgs, p, **kw)
+    R = np.asarray(sc.cameras[0].R, np.float64)
+    assert np.abs(o0["normal"] - st[..., :3].astype(np.float64) @ R).max() < 1e-6
+
+
+def test_compat_switches_change_the_plain_pass():
+    # Q2 and Q3 switched change which candidates win near the borders (Q3:
+    # an out-of-image candidate no longer looks perfect) and in the right
+    # far region (Q2), and the pass still recovers the scene.
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    dmin, dmax = T.depth_range(sc, 0)
+    p = O.make_params(max_iterations=3, depth_min=dmin, depth_max=dmax)
+    cams = T.oracle_cams(sc, ids)
+    imgs = [sc.images[i].numpy() for i in ids]
+    gt = sc.depth[0].numpy()
+    base = O.pm_run(cams, imgs, p, precision=64)
+    for bits in (1, 2):
+        o = O.pm_run(cams, imgs, p, precision=64, compat=bits)
+        changed = float((o["depth"] != base["depth"]).mean())
+        ok = float((np.abs(o["depth"] - gt) / gt < 0.01).mean())
+        assert changed > 0.001, (bits, changed)
+        assert ok > 0.8, (bits, ok)

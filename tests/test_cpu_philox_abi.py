@@ -117,3 +117,42 @@ def test_params_defaults_match_reference():
     assert (p.geom_consistency, p.planar_prior, p.superpixel_filter) == (False, False, False)
     with pytest.raises(AttributeError):
         p.no_such_field = 1
+
+
+def test_ctypes_mirrors_match_the_c_header(tmp_path):
+    """Every struct the Python binding mirrors has the C layout of
+    include/gmvs_amd.h: size and the offset of every field, read from a C
+    program compiled against the header."""
+    import os
+    import shutil
+    import subprocess
+
+    from gmvs_amd import _native as N
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"gmvs_camera": N.CCamera, "gmvs_problem": N.CProblem, "gmvs_params": N.CParams,
+               "gmvs_pm_args": N.CPMArgs, "gmvs_points_args": N.CPointsArgs,
+               "gmvs_kernel_stat": N.CKernelStat}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "gmvs_amd.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'    printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([cc, "-std=c99", f"-I{inc}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            cname, fname, val = ln.split()
+            got[(cname, fname)] = int(val)
+    for cname, cls in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

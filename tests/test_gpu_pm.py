@@ -77,7 +77,7 @@ def test_cost_batch_matches_fp64(kind):
 
 
 def _pm_gpu(sc, ids, params_kw, init_mode=0, geom=False, depths=None, ref_normal=None,
-            prior=None, state=None, skip_filter=False, precision=0):
+            prior=None, state=None, skip_filter=False, precision=0, compat=0):
     N, S = _native()
     dev = torch.device("cuda")
     imgs = [sc.images[i].to(dev).contiguous() for i in ids]
@@ -104,7 +104,7 @@ def _pm_gpu(sc, ids, params_kw, init_mode=0, geom=False, depths=None, ref_normal
     S.run_pm(_ccams(sc, ids), imgs, cp, init_mode=init_mode, geom=geom, rng_view=ids[0],
              planes=planes, costs=costs, depths=dd, ref_normal=rn, prior_planes=pp,
              plane_masks=pm, out_depth=od, out_normal=on, out_cost=oc, skip_filter=skip_filter,
-             precision=precision)
+             precision=precision, compat=compat)
     torch.cuda.synchronize()
     return dict(depth=od.cpu().numpy(), normal=on.cpu().numpy(), cost=oc.cpu().numpy(),
                 planes=planes.cpu().numpy())
@@ -259,3 +259,66 @@ def test_cost_batch_exact_on_bilinear_images():
     gpu = gpu_cost_batch(sc, [0, 1], xy, planes)
     print(f"bilinear images: max cost {gpu.max():.2e}")
     assert np.all(gpu < 1e-5), gpu.max()
+
+
+def test_compat_flags_match_oracle():
+    """The GMVS_COMPAT_* switches (each reference quirk corrected) agree with
+    the oracle's same switches, and change the result."""
+    N, S = _native()
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    cams = T.oracle_cams(sc, ids)
+    imgs = [sc.images[i].numpy() for i in ids]
+    # plain pass: right_far minimum and cost default 2
+    kw = _params(sc)
+    fix = N.COMPAT_FIX_RIGHT_FAR | N.COMPAT_FIX_COST_DEFAULT
+    g = _pm_gpu(sc, ids, kw, compat=fix)
+    o = O.pm_run(cams, imgs, O.make_params(**kw), precision=64, rng_view=0, compat=fix)
+    o0 = O.pm_run(cams, imgs, O.make_params(**kw), precision=64, rng_view=0)
+    agree = T.depth_normal_agreement(g["depth"], g["normal"], o["depth"], o["normal"])
+    changed = 1.0 - T.depth_normal_agreement(o["depth"], o["normal"], o0["depth"], o0["normal"])
+    print(f"compat plain: agreement vs FP64 oracle = {agree:.4f}; pixels changed by the "
+          f"switches = {changed:.4f}")
+    assert agree >= 0.99
+    assert changed > 0.001
+    # prior pass: world normals rotated into the camera frame, selection kept
+    kw = _params(sc, iters=3)
+    o1 = O.pm_run(cams, imgs, O.make_params(**kw), precision=64, rng_view=0)
+    from gmvs_amd.lib.build import planar_initer as PI
+
+    prior_planes, masks, _ = PI.build_prior(_ccams(sc, [0])[0], o1["depth"], o1["cost"], None,
+                                            kw["depth_min"], kw["depth_max"])
+    kw2 = dict(kw, planar_prior=True)
+    fix = N.COMPAT_FIX_PRIOR_NORMALS | N.COMPAT_FIX_PRIOR_SELECTION
+    g = _pm_gpu(sc, ids, kw2, init_mode=2, prior=(prior_planes, masks),
+                state=(o1["planes"].copy(), o1["cost"].copy()), compat=fix)
+    args = dict(precision=64, init_mode=2, prior_planes=prior_planes,
+                plane_masks=masks.astype(np.uint32), planes=o1["planes"], costs=o1["cost"],
+                rng_view=0)
+    o = O.pm_run(cams, imgs, O.make_params(**kw2), compat=fix, **args)
+    o32 = O.pm_run(cams, imgs, O.make_params(**kw2), compat=fix, **dict(args, precision=32))
+    agree = T.depth_normal_agreement(g["depth"], g["normal"], o["depth"], o["normal"])
+    agree32 = T.depth_normal_agreement(o32["depth"], o32["normal"], o["depth"], o["normal"])
+    print(f"compat prior: agreement vs FP64 oracle = {agree:.4f} (oracle FP32: {agree32:.4f})")
+    assert agree >= 0.97
+    assert agree >= agree32 - 0.01
+
+
+def test_compat_prior_normals_round_trip():
+    """With GMVS_COMPAT_FIX_PRIOR_NORMALS an INIT_PRIOR init without a prior
+    reads the world-frame (normal, depth) state and, after GetDepthandNormal,
+    returns it unchanged; without it the normals come back rotated by R^T."""
+    N, S = _native()
+    sc = T.scene()
+    ids = [0, 1, 2]
+    st = T.world_state_facing_camera(sc, 0, seed=4)
+    h, w = st.shape[:2]
+    kw = _params(sc, iters=0)
+    g = _pm_gpu(sc, ids, kw, init_mode=2, state=(st.copy(), np.ones((h, w), np.float32)),
+                skip_filter=True, compat=N.COMPAT_FIX_PRIOR_NORMALS)
+    assert np.abs(g["normal"] - st[..., :3]).max() < 1e-5
+    assert np.abs(g["depth"] / st[..., 3] - 1).max() < 1e-5
+    g0 = _pm_gpu(sc, ids, kw, init_mode=2, state=(st.copy(), np.ones((h, w), np.float32)),
+                 skip_filter=True)
+    R = np.asarray(sc.cameras[0].R, np.float64)
+    assert np.abs(g0["normal"] - st[..., :3].astype(np.float64) @ R).max() < 1e-5

@@ -375,7 +375,8 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+                                   "(views would shard across GPUs with one all-gather of "
+                                   "depth/normal maps per stage; not built)"}))
         return 2
     if args.workload == "scenes4":
         return run_scenes4(args)

@@ -11,7 +11,39 @@ the built library.
 from .version import __version__
 
 __all__ = ["__version__", "run_patch_match", "run_fusion", "initialization",
-           "PatchMatchParams", "Problem"]
+           "PatchMatchParams", "Problem", "install_as_gmvs"]
+
+# The reference's module names (gmvs/__init__.py, gmvs/consistency.py:34,
+# gmvs/planar_init.py:54) and the modules of this package that replace them.
+_REFERENCE_MODULES = (
+    ("gmvs", ""),
+    ("gmvs.src", ".src"),
+    ("gmvs.lib", ".lib"),
+    ("gmvs.lib.build", ".lib.build"),
+    ("gmvs.lib.build.planar_initer", ".lib.build.planar_initer"),
+    ("gmvs.consistency", ".consistency"),
+    ("gmvs.planar_init", ".planar_init"),
+    ("gmvs.primitives", ".primitives"),
+    ("gmvs.scripts", ".scripts"),
+    ("gmvs.scripts.launch", ".scripts.launch"),
+)
+
+
+def install_as_gmvs() -> None:
+    """Registers this package under the reference's import names, so code
+    written against the reference (``import gmvs.src as _C``, ``from
+    gmvs.lib.build import planar_initer``, ``from gmvs import
+    run_patch_match``) runs on this implementation unchanged. Raises
+    ImportError if another package is already imported as ``gmvs``."""
+    import importlib
+    import sys
+
+    cur = sys.modules.get("gmvs")
+    if cur is not None and cur is not sys.modules[__name__]:
+        raise ImportError(f"a different gmvs package is already imported ({cur!r})")
+    for ref_name, rel in _REFERENCE_MODULES:
+        mod = sys.modules[__name__] if not rel else importlib.import_module(rel, __name__)
+        sys.modules[ref_name] = mod
 
 
 def __getattr__(name):

@@ -52,3 +52,28 @@ def test_gray_round_trip(tmp_path):
     p = tmp_path / "0000.png"
     gio.write_gray(str(p), img)
     assert np.array_equal(gio.read_gray(str(p)), img)
+
+
+def test_install_as_gmvs_maps_the_reference_imports():
+    """Reference code imports this implementation unchanged after
+    gmvs_amd.install_as_gmvs() (run in a child interpreter: sys.modules)."""
+    import subprocess
+    import sys
+
+    code = (
+        "import gmvs_amd\n"
+        "gmvs_amd.install_as_gmvs()\n"
+        "import gmvs.src as _C\n"
+        "from gmvs.lib.build import planar_initer\n"
+        "from gmvs import run_patch_match, run_fusion, initialization, Problem, PatchMatchParams\n"
+        "import gmvs.scripts.launch\n"
+        "import gmvs_amd.src, gmvs_amd.lib.build.planar_initer, gmvs_amd.consistency\n"
+        "assert _C is gmvs_amd.src\n"
+        "assert planar_initer is gmvs_amd.lib.build.planar_initer\n"
+        "assert run_patch_match is gmvs_amd.consistency.run_patch_match\n"
+        "assert hasattr(_C, 'PatchMatcher') and hasattr(_C, 'Fuser')\n"
+        "print('ok')\n")
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stderr

@@ -28,7 +28,10 @@ INIT_PRIOR = 2
 PREC_FP32 = 0
 PREC_FP16 = 1
 PREC_BF16 = 2
-PRECISIONS = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16}
+# host-side choice only (gmvs_amd.src.resolve_precision): FP16 cells when they
+# hold the images exactly (results identical to PREC_FP32), else FP32 cells
+PREC_AUTO = -1
+PRECISIONS = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16, "auto": PREC_AUTO}
 
 # gmvs_pm_args.compat bits (GMVS_COMPAT_*): 0 reproduces the reference's quirks
 COMPAT_FIX_RIGHT_FAR = 1

@@ -57,7 +57,7 @@ class MultiViewPipeline:
         self.labels = labels
         self.params = params if params is not None else S.PatchMatchParams()
         self.prior_threads = prior_threads
-        self.precision = precision
+        self.precision = S.resolve_precision(precision, self.images)
         self.batched = batched
         self.tile_cols = tile_cols
         self.groups = max(1, min(groups, self.n))

@@ -172,6 +172,41 @@ def _device() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def fp16_cells_exact(images) -> bool:
+    """True when FP16 cells hold every bilinear cell of these images exactly:
+    integer intensities in [0, 1024] (8-bit imagery is), so the cell's value,
+    its x and y differences and its cross difference (|.| <= 2048) are
+    integers FP16 represents exactly, and GMVS_PREC_FP16 -- whose bilinear
+    sample widens the FP16 operands into FP32 FMAs -- returns the FP32
+    path's results bit for bit with half the gather bytes."""
+    for im in images:
+        t = im if isinstance(im, torch.Tensor) else torch.as_tensor(np.asarray(im))
+        t = t.float()
+        if t.numel() == 0:
+            continue
+        if not bool(torch.isfinite(t).all()) or float(t.min()) < 0.0 or float(t.max()) > 1024.0:
+            return False
+        if not bool((t == torch.round(t)).all()):
+            return False
+    return True
+
+
+def resolve_precision(precision, images) -> int:
+    """GMVS_PREC_* for a precision given as an int or a name ("fp32", "fp16",
+    "bf16", "auto"); "auto" (N.PREC_AUTO) is FP16 cells where
+    fp16_cells_exact(images), else FP32 cells: the FP32 path's results
+    either way."""
+    if isinstance(precision, str):
+        if precision not in N.PRECISIONS:
+            raise ValueError(f"unknown precision {precision!r}")
+        precision = N.PRECISIONS[precision]
+    if precision == N.PREC_AUTO:
+        return N.PREC_FP16 if fp16_cells_exact(images) else N.PREC_FP32
+    if precision not in (N.PREC_FP32, N.PREC_FP16, N.PREC_BF16):
+        raise ValueError(f"unknown precision {precision!r}")
+    return int(precision)
+
+
 def pm_params_for_problem(params: PatchMatchParams, ref_cam: Camera, num_images: int) -> N.CParams:
     """The per-call parameter overrides of run_patch_match (src 1377-1380):
     num_images = 1 + sources, depth range = camera range x 0.6 / x 1.2."""

@@ -77,3 +77,24 @@ def test_install_as_gmvs_maps_the_reference_imports():
     out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
                          timeout=300)
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stderr
+
+
+def test_fp16_cells_exact_and_auto_precision():
+    import numpy as np
+    import torch
+
+    from gmvs_amd import _native as N
+    from gmvs_amd import src as S
+
+    rng = np.random.default_rng(0)
+    u8 = [torch.from_numpy(rng.integers(0, 256, (12, 16)).astype(np.float32)) for _ in range(3)]
+    assert S.fp16_cells_exact(u8)
+    assert S.resolve_precision("auto", u8) == N.PREC_FP16
+    frac = [u8[0] + 0.5]
+    assert not S.fp16_cells_exact(frac)
+    assert S.resolve_precision(N.PREC_AUTO, frac) == N.PREC_FP32
+    assert not S.fp16_cells_exact([u8[0] * 8])      # beyond 1024
+    assert not S.fp16_cells_exact([u8[0] - 1.0])    # negative
+    assert S.resolve_precision("bf16", u8) == N.PREC_BF16
+    with pytest.raises(ValueError):
+        S.resolve_precision("fp8", u8)

@@ -119,3 +119,28 @@ def test_pipeline_batched_equals_per_view():
         res.append([t.cpu().numpy() for t in (s1.depth, s1.normal, s2.depth, s2.normal, s2.cost)])
     for a, b in zip(*res):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_pipeline_auto_precision_equals_fp32():
+    """precision="auto" picks FP16 cells for 8-bit imagery, which hold it
+    exactly: the whole pipeline (prior, geometric pass) returns the FP32
+    path's bits."""
+    N, S = _native()
+    from gmvs_amd.pipeline import MultiViewPipeline, cameras_from_synthetic
+
+    sc = make_scene(240, 180, 5, 5, seed=3407 + 11)
+    assert S.fp16_cells_exact(sc.images)
+    cams = cameras_from_synthetic(sc.cameras)
+    pairs = [p[:4] for p in sc.pairs]
+    labels = [voronoi_labels(240, 180, 40, seed=3407 + 30 + i) for i in range(5)]
+    res = []
+    for prec in ("auto", N.PREC_FP32):
+        pipe = MultiViewPipeline(sc.images, cams, pairs, labels=labels, prior_threads=4,
+                                 precision=prec)
+        assert pipe.precision == (N.PREC_FP16 if prec == "auto" else N.PREC_FP32)
+        s1 = pipe.first_stage(iterations=2, planar_prior=True, superpixel_filter=True)
+        s2 = pipe.geom_pass(s1, iterations=1)
+        torch.cuda.synchronize()
+        res.append([t.cpu().numpy() for t in (s1.depth, s1.normal, s2.depth, s2.normal, s2.cost)])
+    for a, b in zip(*res):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -494,8 +494,10 @@ def main() -> int:
         vt = time.perf_counter() - t0
         vval = ph_step * args.steps / vt / 1e6
         vfused = int((vfd > 0).sum().item())
+        # the fused depth map against the FP32 batched run's, bit for bit
+        same = bool(torch.equal(vfd.contiguous().view(torch.int32), fd.contiguous().view(torch.int32)))
         variants[name] = {"value": round(vval, 2), "ms_per_step": round(vt / args.steps * 1e3, 2),
-                          "fused_points": vfused}
+                          "fused_points": vfused, "fused_depth_bits_equal_fp32": same}
         log(f"{name}: {variants[name]}")
         del vp
     plain = variants.get("fp32_per_view", {}).get("value")

@@ -1,6 +1,6 @@
 // Oracle for the gmvs fusion step -- TEST INFRASTRUCTURE ONLY.
 //
-// Sequential FP32 restatement of batch_fusion_kernel
+// FP32 restatement of batch_fusion_kernel
 // (gmvs/src/fusion.cu:61-182), the bit-exact reference that the HIP fusion
 // kernel's masks and fused point counts are compared with. Built with
 // -ffp-contract=off so every product and sum is rounded exactly as written;
@@ -16,12 +16,15 @@
 //   * pixel rounding int(x + 0.5f) is taken as in range when
 //     -1 < x + 0.5f < n (truncation toward zero, fusion.cu:112-113);
 //   * scatter collisions: the highest reference pixel index wins, which is
-//     what this sequential loop's last writer is (fusion.cu:164-171);
+//     what the in-order write phase's last writer is (fusion.cu:164-171);
 //   * masks[ref] is read for the whole view before any mask of this view is
 //     written (the reference only writes masks of source views).
 #include <cmath>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "oracle_common.h"
 
@@ -86,11 +89,22 @@ int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const
     const int nsrc = num_images - 1;
     const OCamera& rc = cams[0];
     std::vector<uint8_t> ref_mask(masks + size_t(ref_id) * hw, masks + size_t(ref_id) * hw + hw);
-    std::vector<int32_t> used(size_t(nsrc) * 2);
-    int64_t consistent_pixels = 0;
+    // Phase 1, every reference pixel on its own (it reads only the inputs and
+    // this view's mask as it was on entry): the consistency checks, the
+    // fused point and the source pixels it used. Phase 2, in pixel order: the
+    // scatter (so the highest pixel index is the last writer) and the source
+    // masks. The arithmetic of each pixel is the sequential loop's.
+    std::vector<uint8_t> ok(hw, 0);
+    std::vector<int32_t> dest(hw, -1);
+    std::vector<float> pdep(hw), pnrm(hw * 3);
+    std::vector<int16_t> used(hw * size_t(nsrc) * 2);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
     for (int r = 0; r < h; ++r) {
         for (int cc = 0; cc < w; ++cc) {
             const size_t ri = size_t(r) * w + cc;
+            int16_t* u_ri = &used[ri * size_t(nsrc) * 2];
             if (ref_mask[ri]) continue;
             const float rd = depths[0][ri];
             const P3 rn{normals[0][3 * ri], normals[0][3 * ri + 1], normals[0][3 * ri + 2]};
@@ -99,8 +113,8 @@ int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const
             P3 cp = rp, cn = rn;
             int num = 0;
             for (int j = 0; j < nsrc; ++j) {
-                used[2 * j] = -1;
-                used[2 * j + 1] = -1;
+                u_ri[2 * j] = -1;
+                u_ri[2 * j + 1] = -1;
                 const OCamera& sc = cams[j + 1];
                 float u, v, pd;
                 project(rp, sc, u, v, pd);
@@ -127,13 +141,13 @@ int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const
                     cn.x += sn.x;
                     cn.y += sn.y;
                     cn.z += sn.z;
-                    used[2 * j] = scol;
-                    used[2 * j + 1] = sr;
+                    u_ri[2 * j] = int16_t(scol);
+                    u_ri[2 * j + 1] = int16_t(sr);
                     ++num;
                 }
             }
             if (num < geom_consistent) continue;
-            ++consistent_pixels;
+            ok[ri] = 1;
             const float den = float(num) + 1.0f;
             cp.x /= den;
             cp.y /= den;
@@ -141,20 +155,33 @@ int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const
             cn.x /= den;
             cn.y /= den;
             cn.z /= den;
-            float pu, pv, pdep;
-            project(cp, rc, pu, pv, pdep);
+            float pu, pv, pdp;
+            project(cp, rc, pu, pv, pdp);
             int pr, pc;
-            if (round_in(pv, h, &pr) && round_in(pu, w, &pc) && pdep > 0.001f) {
-                const size_t pi = size_t(pr) * w + pc;
-                out_depth[pi] = pdep;
-                out_normal[3 * pi] = cn.x;
-                out_normal[3 * pi + 1] = cn.y;
-                out_normal[3 * pi + 2] = cn.z;
+            if (round_in(pv, h, &pr) && round_in(pu, w, &pc) && pdp > 0.001f) {
+                dest[ri] = int32_t(size_t(pr) * w + pc);
+                pdep[ri] = pdp;
+                pnrm[3 * ri] = cn.x;
+                pnrm[3 * ri + 1] = cn.y;
+                pnrm[3 * ri + 2] = cn.z;
             }
-            for (int j = 0; j < nsrc; ++j) {
-                if (used[2 * j] == -1) continue;
-                masks[size_t(src_ids[j]) * hw + size_t(used[2 * j + 1]) * w + used[2 * j]] = 1;
-            }
+        }
+    }
+    int64_t consistent_pixels = 0;
+    for (size_t ri = 0; ri < hw; ++ri) {
+        if (!ok[ri]) continue;
+        ++consistent_pixels;
+        if (dest[ri] >= 0) {
+            const size_t pi = size_t(dest[ri]);
+            out_depth[pi] = pdep[ri];
+            out_normal[3 * pi] = pnrm[3 * ri];
+            out_normal[3 * pi + 1] = pnrm[3 * ri + 1];
+            out_normal[3 * pi + 2] = pnrm[3 * ri + 2];
+        }
+        const int16_t* u_ri = &used[ri * size_t(nsrc) * 2];
+        for (int j = 0; j < nsrc; ++j) {
+            if (u_ri[2 * j] == -1) continue;
+            masks[size_t(src_ids[j]) * hw + size_t(u_ri[2 * j + 1]) * w + u_ri[2 * j]] = 1;
         }
     }
     return consistent_pixels;

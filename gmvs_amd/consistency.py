@@ -33,8 +33,11 @@ def run_patch_match(
     problems: List[Problem],
     verbose: bool = False,
     first_stage: bool = True,
+    precision="fp32",
 ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
-    """consistency.py:27-138."""
+    """consistency.py:27-138. ``precision`` (not in the reference): "fp32"
+    (the reference's arithmetic), "fp16", "bf16" or "auto" (FP16 cells where
+    they hold the images exactly: the FP32 results)."""
     from . import src as _C
 
     num_imgs = len(problems)
@@ -60,6 +63,7 @@ def run_patch_match(
         costs.append(cost)
 
     matcher = _C.PatchMatcher()
+    matcher.precision = precision
     matcher.params.geom_consistency = True
     matcher.params.max_iterations = 2
 

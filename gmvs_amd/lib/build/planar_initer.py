@@ -21,7 +21,7 @@ from ... import _native as N
 from ... import io as gio
 from ... import post as _post
 from ...src import (Camera, PatchMatchParams, Problem, _Workspace, _device, load_camera,
-                    load_problem, run_pm)
+                    load_problem, resolve_precision, run_pm)
 
 __all__ = ["PlanarInit", "PatchMatchParams", "Problem", "Camera", "load_camera", "load_problem",
            "fusion_planar_mask", "filter_by_var_map", "sliding_window_normal", "build_prior",
@@ -107,6 +107,9 @@ class PlanarInit:
     def __init__(self, verbose: bool = False):
         self.verbose = bool(verbose)
         self.params = PatchMatchParams()
+        # GMVS_PREC_* or a name ("fp32", "fp16", "bf16", "auto"); not in the
+        # reference, which computes in FP32 (the default)
+        self.precision = N.PREC_FP32
         self._images: List[np.ndarray] = []
         self._cameras: List[Camera] = []
         self._superpixel: Optional[np.ndarray] = None
@@ -176,7 +179,8 @@ class PlanarInit:
                init_mode=N.INIT_PRIOR if prior else N.INIT_RANDOM, geom=False, rng_view=self.rng_view,
                planes=self._planes, costs=self._costs,
                prior_planes=self._prior[0] if prior else None,
-               plane_masks=self._prior[1] if prior else None, workspace=self._ws)
+               plane_masks=self._prior[1] if prior else None, workspace=self._ws,
+               precision=resolve_precision(self.precision, self._dev_images))
         torch.cuda.synchronize()
         if self.verbose:
             for it in range(self.params.max_iterations):

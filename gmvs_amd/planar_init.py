@@ -38,9 +38,11 @@ def initialization(
     superpixel_dir: Optional[str] = None,
     input_depth_normal_dir: Optional[str] = None,
     max_iterations: Optional[int] = None,
+    precision="fp32",
 ) -> None:
     """``max_iterations`` (not in the reference) overrides the default of
-    PatchMatchParams (3) for both passes."""
+    PatchMatchParams (3) for both passes; ``precision`` (not in the
+    reference) as for consistency.run_patch_match."""
     ref_image_id = problem.ref_image_id
     save_dir = os.path.join(output_dir, f"{ref_image_id:04}")
     os.makedirs(save_dir, exist_ok=True)
@@ -54,6 +56,7 @@ def initialization(
     from .lib.build import planar_initer as _C
 
     initer = _C.PlanarInit(verbose)
+    initer.precision = precision
     initer.rng_view = ref_image_id
     height, width = _load_views(initer, result_dir, problem)
     if not no_init and superpixel_dir:

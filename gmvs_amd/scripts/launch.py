@@ -46,6 +46,9 @@ def get_args(argv=None) -> argparse.Namespace:
     parser.add_argument("--verbose", "-v", action="store_true", help="verbose description")
     parser.add_argument("--max_iterations", type=int, default=None,
                         help="first-stage iterations (default: PatchMatchParams' 3)")
+    parser.add_argument("--precision", default="fp32", choices=("fp32", "fp16", "bf16", "auto"),
+                        help="source samples: fp32 (the reference's arithmetic), fp16/bf16 "
+                             "cells, or auto (fp16 cells where they hold the images exactly)")
     return parser.parse_args(argv)
 
 
@@ -76,6 +79,7 @@ def main(argv=None) -> int:
             superpixel_dir=args.superpixel_dir,
             input_depth_normal_dir=args.input_depth_normal_dir,
             max_iterations=args.max_iterations,
+            precision=args.precision,
         )
 
     valid = None
@@ -86,6 +90,7 @@ def main(argv=None) -> int:
             problems=problems,
             verbose=args.verbose,
             first_stage=(i == 0),
+            precision=args.precision,
         )
     if args.geom_iters > 0:
         valid = run_fusion(depths, normals, result_root, problems,

@@ -300,6 +300,9 @@ class PatchMatcher:
 
     def __init__(self):
         self.params = PatchMatchParams()
+        # GMVS_PREC_* or a name ("fp32", "fp16", "bf16", "auto"); not in the
+        # reference, which computes in FP32 (the default)
+        self.precision = N.PREC_FP32
         self._problems: List[Problem] = []
         self._cameras: List[Camera] = []
         self.images_host = self.depths_host = self.normals_host = self.costs_host = None
@@ -352,7 +355,7 @@ class PatchMatcher:
         run_pm(cams, [images[i] for i in ids], params, init_mode=N.INIT_DEPTH_NORMAL, geom=True,
                rng_view=ref, planes=planes, costs=costs, depths=[depths[i] for i in ids],
                ref_normal=normals[ref], out_depth=out_d, out_normal=out_n, out_cost=out_c,
-               workspace=self._ws)
+               workspace=self._ws, precision=resolve_precision(self.precision, images))
         if verbose:
             for it in range(self.params.max_iterations):
                 print(f"iteration: {it}")

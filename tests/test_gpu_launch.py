@@ -59,3 +59,28 @@ def test_launch_end_to_end(tmp_path):
             rel = np.abs(dn[..., 0][ok] - gt[ok]) / gt[ok]
             assert np.mean(rel < 0.02) > 0.8, i
     assert fused_any > 0.3 * H * W
+
+
+def test_launch_auto_precision_writes_the_fp32_files(tmp_path):
+    """--precision auto on 8-bit PNGs (FP16 cells, which hold them exactly)
+    writes the same stage and fused files, bit for bit, as the default FP32
+    run."""
+    from gmvs_amd.scripts import launch
+
+    outs = {}
+    for prec in ("fp32", "auto"):
+        root, sp = str(tmp_path / prec / "scene"), str(tmp_path / prec / "superpixels")
+        _write_dataset(root, sp)
+        rc = launch.main(["-rd", root, "-sd", sp, "--max_iterations", "2", "-gi", "1", "-gc", "2",
+                          "--precision", prec])
+        assert rc == 0
+        files = {}
+        for i in range(V):
+            d = os.path.join(root, "mvs", f"{i:04}")
+            for name in ("depth", "normal", "cost", "depth_geom", "normal_geom", "cost_geom"):
+                files[(i, name)] = np.load(os.path.join(d, name + ".npy"))
+            files[(i, "fused")] = np.load(os.path.join(root, "mvs", "depth_normal", f"{i:04}.npy"))
+        outs[prec] = files
+    for key, a in outs["fp32"].items():
+        b = outs["auto"][key]
+        assert a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8)), key

@@ -937,44 +937,62 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
             }
         }
 
-        // ---- candidate final costs (src 983-1009)
+        // ---- candidate final costs (src 983-1009) and the current
+        // hypothesis re-costed with the new view weights (its NCCs come from
+        // pm_candidates, row 8 of the record). Plain and prior passes walk
+        // the selected views once and read the view's 9 costs together; the
+        // geometric pass adds a reprojection term per (candidate, view).
+        // Every final cost is its weighted sum times one reciprocal of the
+        // weight sum: a candidate that is the current plane must compare
+        // equal to it (the adoption test is strict), and two divisions by
+        // wsum may be lowered to different approximate sequences.
         float fc[8];
+        const float rws = 1.0f / wsum;
+        float4 cur = planes[c];
+        float cnow;
+        if (!P.geom) {
+            float sk[9];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const bool fk = (flags >> k) & 1u;
-            float4 cp = make_float4(0.f, 0.f, 0.f, 1.f);
-            if (fk && P.geom) cp = cplane[(size_t)k * NA + aidx];
-            float s = 0.f;
+            for (int k = 0; k < 9; ++k) sk[k] = 0.f;
             for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
                 const int v = __ffs(mm) - 1;
                 const float wv = nib_weight(wnib, v);
-                const float x = ca_entry(ca, k, v, nv, NA, aidx);
-                if (P.geom && fk)
-                    s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cp, fpx, fpy));
-                else
-                    s += wv * x;
+                float xv[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) xv[k] = ca_entry(ca, k, v, nv, NA, aidx);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) sk[k] = fmaf(wv, xv[k], sk[k]);
             }
-            fc[k] = s / wsum;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) fc[k] = sk[k] * rws;
+            cnow = sk[8] * rws;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const bool fk = (flags >> k) & 1u;
+                float4 cp = make_float4(0.f, 0.f, 0.f, 1.f);
+                if (fk) cp = cplane[(size_t)k * NA + aidx];
+                float sk = 0.f;
+                for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+                    const int v = __ffs(mm) - 1;
+                    const float wv = nib_weight(wnib, v);
+                    const float x = ca_entry(ca, k, v, nv, NA, aidx);
+                    const float g = fk ? x + 0.1f * geom_cost(rc, s_views[v], cp, fpx, fpy) : x;
+                    sk = fmaf(wv, g, sk);
+                }
+                fc[k] = sk * rws;
+            }
+            float sk = 0.f;
+            for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+                const int v = __ffs(mm) - 1;
+                const float x = ca_entry(ca, 8, v, nv, NA, aidx);
+                const float wv = nib_weight(wnib, v);
+                sk = fmaf(wv, x + 0.1f * geom_cost(rc, s_views[v], cur, fpx, fpy), sk);
+            }
+            cnow = sk * rws;
         }
         const int kmin = argmin_last8(fc);
 
-        // The current hypothesis re-costed with the new view weights; its
-        // NCCs come from pm_candidates (row 8 of the record).
-        float4 cur = planes[c];
-        float cnow;
-        {
-            float s = 0.f;
-            for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
-                const int v = __ffs(mm) - 1;
-                const float x = ca[((size_t)8 * nv + v) * NA + aidx];
-                const float wv = nib_weight(wnib, v);
-                if (P.geom)
-                    s += wv * (x + 0.1f * geom_cost(rc, s_views[v], cur, fpx, fpy));
-                else
-                    s += wv * x;
-            }
-            cnow = s / wsum;
-        }
         float cost = cnow;
         uint32_t selc = sel[c];
         float depth_now = depth_from_plane(rc, cur, fpx, fpy);

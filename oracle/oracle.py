@@ -13,6 +13,9 @@ this module, and only as the checker (or the timed CPU baseline). The product
                        sequential twins of the post-fusion utilities
     gather_points(...) numpy restatement of the post-fusion point gather
     philox(...)        the oracle's own Philox4x32-10 block
+    t_*(...)           the decision steps of one half-sweep pixel update on
+                       hand-built inputs (candidate search, view selection,
+                       final costs, adoption, refinement), FP64
 """
 from __future__ import annotations
 
@@ -91,6 +94,16 @@ def _load():
     lib.oracle_filter_by_var_map.restype = None
     lib.oracle_fusion_planar_mask.argtypes = [vp, vp, vp, i32, i32, i32, f32, f32, f32, vp]
     lib.oracle_fusion_planar_mask.restype = None
+    d = ctypes.c_double
+    lib.oracle_t_candidates.argtypes = [vp, i32, i32, i32, i32, ctypes.c_uint32, vp, vp]
+    lib.oracle_t_select_views.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp, vp, vp]
+    lib.oracle_t_final_costs.argtypes = [vp, i32, vp, vp, d, vp, vp]
+    lib.oracle_t_adopt.argtypes = [i32, vp, vp, vp, d, d, d]
+    lib.oracle_t_refine_hypotheses.argtypes = [d, vp, d, vp, d, vp, vp, vp]
+    lib.oracle_t_refine_hypotheses.restype = None
+    lib.oracle_t_refine_adopt.argtypes = [i32, vp, vp, vp, d, d, vp, vp]
+    lib.oracle_t_prior_score.argtypes = [d, d, d, vp, vp, d, d]
+    lib.oracle_t_prior_score.restype = d
     return lib
 
 
@@ -320,3 +333,86 @@ def gather_points(depth_normals, poses, intrinsic, height: int, width: int):
         pts_all.append(pn)
         pts_f.append(pn[vm])
     return np.concatenate(pts_all), np.concatenate(pts_f), np.stack(maps)
+
+
+# ---- decision steps of the half-sweep (FP64), for known-answer tests ----
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def t_candidates(costs: np.ndarray, px: int, py: int, compat: int = 0):
+    """Candidate search over a [h, w] cost grid: (flag[8] bool, pos[8] flat index or -1)."""
+    c = _f64(costs)
+    h, w = c.shape
+    flag = np.zeros(8, np.int32)
+    pos = np.zeros(8, np.int32)
+    rc = lib().oracle_t_candidates(_ptr(c), h, w, px, py, compat, _ptr(flag), _ptr(pos))
+    assert rc == 0, rc
+    return flag.astype(bool), pos
+
+
+def t_select_views(ca: np.ndarray, flag, nbsel, it: int, draws):
+    """View selection from an [8, nv] cost table: (cdf[nv], counts[nv], tsel, wn)."""
+    ca = _f64(ca)
+    nv = ca.shape[1]
+    fl = _i32(flag)
+    nb = np.ascontiguousarray(nbsel, np.uint32)
+    dr = _f64(draws)
+    assert dr.shape == (15,)
+    cdf = np.zeros(nv)
+    wv = np.zeros(nv)
+    tsel = np.zeros(1, np.uint32)
+    wn = np.zeros(1)
+    rc = lib().oracle_t_select_views(_ptr(ca), nv, _ptr(fl), _ptr(nb), it, _ptr(dr), _ptr(cdf),
+                                     _ptr(wv), _ptr(tsel), _ptr(wn))
+    assert rc == 0, rc
+    return cdf, wv, int(tsel[0]), float(wn[0])
+
+
+def t_final_costs(ca: np.ndarray, flag, wv, wn: float, geom: Optional[np.ndarray] = None):
+    ca = _f64(ca)
+    nv = ca.shape[1]
+    fl = _i32(flag)
+    w_ = _f64(wv)
+    g = None if geom is None else _f64(geom)
+    fc = np.zeros(8)
+    rc = lib().oracle_t_final_costs(_ptr(ca), nv, _ptr(fl), _ptr(w_), float(wn), _ptr(g), _ptr(fc))
+    assert rc == 0, rc
+    return fc
+
+
+def t_adopt(prior: bool, score, flag, dk, now: float, dmin: float, dmax: float) -> int:
+    s, fl, d = _f64(score), _i32(flag), _f64(dk)
+    return int(lib().oracle_t_adopt(1 if prior else 0, _ptr(s), _ptr(fl), _ptr(d), float(now),
+                                    float(dmin), float(dmax)))
+
+
+def t_refine_hypotheses(drand, nrand, dcur, ncur, dpert, npert):
+    nr, nc, npt = _f64(nrand), _f64(ncur), _f64(npert)
+    depths = np.zeros(5)
+    normals = np.zeros((5, 3))
+    lib().oracle_t_refine_hypotheses(float(drand), _ptr(nr), float(dcur), _ptr(nc), float(dpert),
+                                     _ptr(npt), _ptr(depths), _ptr(normals))
+    return depths, normals
+
+
+def t_refine_adopt(pm: bool, tc, db, sc, dmin: float, dmax: float, cost: float,
+                   restricted: float):
+    """Returns (last adopted hypothesis or -1, final cost, final restricted score)."""
+    t, d, s = _f64(tc), _f64(db), _f64(sc)
+    c = np.array([cost], np.float64)
+    r = np.array([restricted], np.float64)
+    k = lib().oracle_t_refine_adopt(1 if pm else 0, _ptr(t), _ptr(d), _ptr(s), float(dmin),
+                                    float(dmax), _ptr(c), _ptr(r))
+    return int(k), float(c[0]), float(r[0])
+
+
+def t_prior_score(cost, depth, depth_prior, n_prior, n, dmin, dmax) -> float:
+    a, b = _f64(n_prior), _f64(n)
+    return float(lib().oracle_t_prior_score(float(cost), float(depth), float(depth_prior), _ptr(a),
+                                            _ptr(b), float(dmin), float(dmax)))

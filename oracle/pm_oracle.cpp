@@ -376,6 +376,259 @@ int argmax_last(const T* c, int n) {
     return k;
 }
 
+// ------------------------------------------------ decision steps of a sweep ---
+// The per-pixel update of a half-sweep is split into the reference's steps so
+// that each step can be checked on its own with hand-built inputs (the
+// oracle_t_* hooks at the end of this file call exactly these functions).
+
+// Candidate search of checkerboard_propagation (lib 862-1104, src 720-913)
+// over the cost grid cs (the half-sweep snapshot). flag[k] and pos[k] use the
+// reference's order: 0 up_near, 1 up_far, 2 down_near, 3 down_far,
+// 4 left_near, 5 left_far, 6 right_near, 7 right_far. Far probes: offset 3,
+// then 3 + 2i (i = 1..10) while inside the image; near probes: offset 1, then
+// the V pattern (1 + i) further out and i to either side (i = 0..2). Every
+// branch keeps the first strict minimum except right_far, which keeps the
+// first strict maximum (lib 954, src 794) unless OCOMPAT_FIX_RIGHT_FAR.
+template <class T>
+void candidate_search(const T* cs, int h, int w, int px, int py, uint32_t compat, bool flag[8],
+                      int pos[8]) {
+    const int c = py * w + px;
+    for (int k = 0; k < 8; ++k) {
+        flag[k] = false;
+        pos[k] = -1;
+    }
+    // Far probes along one axis: step = +-w (vertical) or +-1 (horizontal);
+    // room(i) says whether the probe 3 + 2i cells out is inside the image.
+    auto far = [&](int k, int step, auto room, bool keep_max) {
+        flag[k] = true;
+        int best = c + 3 * step;
+        T m = cs[best];
+        for (int i = 1; i < 11; ++i)
+            if (room(i)) {
+                const int q = c + (3 + 2 * i) * step;
+                const bool better = keep_max ? (m < cs[q]) : (cs[q] < m);
+                if (better) {
+                    m = cs[q];
+                    best = q;
+                }
+            }
+        pos[k] = best;
+    };
+    if (py > 2) far(1, -w, [&](int i) { return py > 2 + 2 * i; }, false);
+    if (py < h - 3) far(3, w, [&](int i) { return py < h - 3 - 2 * i; }, false);
+    if (px > 2) far(5, -1, [&](int i) { return px > 2 + 2 * i; }, false);
+    if (px < w - 3)
+        far(7, 1, [&](int i) { return px < w - 3 - 2 * i; }, !(compat & OCOMPAT_FIX_RIGHT_FAR));
+    // Near probes: out = the outward step, side = the sideways step; ok_out(i)
+    // and ok_lo(i) / ok_hi(i) are the reference's bounds tests.
+    auto near = [&](int k, int out, int side, auto ok_lo, auto ok_hi) {
+        flag[k] = true;
+        const int base = c + out;
+        int best = base;
+        T m = cs[best];
+        for (int i = 0; i < 3; ++i) {
+            if (ok_lo(i)) {
+                const int q = base + (1 + i) * out - i * side;
+                if (cs[q] < m) {
+                    m = cs[q];
+                    best = q;
+                }
+            }
+            if (ok_hi(i)) {
+                const int q = base + (1 + i) * out + i * side;
+                if (cs[q] < m) {
+                    m = cs[q];
+                    best = q;
+                }
+            }
+        }
+        pos[k] = best;
+    };
+    if (py > 0)
+        near(0, -w, 1, [&](int i) { return py > 1 + i && px > i; },
+             [&](int i) { return py > 1 + i && px < w - 1 - i; });
+    if (py < h - 1)
+        near(2, w, 1, [&](int i) { return py < h - 2 - i && px > i; },
+             [&](int i) { return py < h - 2 - i && px < w - 1 - i; });
+    if (px > 0)
+        near(4, -1, w, [&](int i) { return px > 1 + i && py > i; },
+             [&](int i) { return px > 1 + i && py < h - 1 - i; });
+    if (px < w - 1)
+        near(6, 1, w, [&](int i) { return px < w - 2 - i && py > i; },
+             [&](int i) { return px < w - 2 - i && py < h - 1 - i; });
+}
+
+// Joint view selection (lib 1114-1183, src 915-981) and transform_PDF_to_CDF
+// (src 79-92). ca is the 8 x nv cost table (candidate-major), nbsel the
+// selected-view masks of the up, down, left and right neighbours (used only
+// where the near flags 0, 2, 4, 6 are set), draws the 15 uniforms in (0, 1].
+// Outputs: cdf[nv], wv[nv] (draw counts), *tsel (views with a count), *wn
+// (the sum of the counts).
+template <class T>
+void select_views(const T* ca, int nv, const bool flag[8], const uint32_t nbsel[4], int iter,
+                  const T draws[15], T* cdf, T* wv, uint32_t* tsel, T* wn) {
+    T prior_v[32], prob[32];
+    for (int v = 0; v < nv; ++v) prior_v[v] = T(0);
+    for (int i = 0; i < 4; ++i)
+        if (flag[2 * i])
+            for (int v = 0; v < nv; ++v) prior_v[v] += ((nbsel[i] >> v) & 1u) ? T(0.9) : T(0.1);
+    const T tau = T(0.8) * std::exp(T(iter) * T(iter) / T(-90));
+    for (int v = 0; v < nv; ++v) {
+        T count = 0, tmpw = 0;
+        int cf = 0;
+        for (int k = 0; k < 8; ++k) {
+            const T x = ca[size_t(k) * nv + v];
+            if (x < tau) {
+                tmpw += std::exp(x * x / T(-0.18));
+                count += T(1);
+            }
+            if (x > T(1.2)) ++cf;
+        }
+        prob[v] = T(0);
+        if (count > T(2) && cf < 3)
+            prob[v] = tmpw / count;
+        else if (cf < 3)
+            prob[v] = std::exp(tau * tau / T(-0.32));
+        prob[v] = prob[v] * prior_v[v];
+    }
+    T sum = 0;
+    for (int v = 0; v < nv; ++v) sum += prob[v];
+    const T inv = T(1) / sum;
+    T cum = 0;
+    for (int v = 0; v < nv; ++v) {
+        cum += prob[v] * inv;
+        cdf[v] = cum;
+        wv[v] = T(0);
+    }
+    for (int d = 0; d < 15; ++d) {
+        const T rp = draws[d] - T(FLT_EPSILON);
+        for (int v = 0; v < nv; ++v)
+            if (cdf[v] > rp) {
+                wv[v] += T(1);
+                break;
+            }
+    }
+    *tsel = 0;
+    *wn = T(0);
+    for (int v = 0; v < nv; ++v)
+        if (wv[v] > T(0)) {
+            *tsel |= 1u << v;
+            *wn += wv[v];
+        }
+}
+
+// Final costs of the 8 candidates (src 983-1002): the draw-count-weighted
+// mean over the drawn views; with the geometric term (gt non-null, 8 x nv)
+// flagged candidates add 0.1 * geom per view, unflagged ones add nothing.
+template <class T>
+void final_costs(const T* ca, int nv, const bool flag[8], const T* wv, T wn, const T* gt,
+                 T fc[8]) {
+    for (int k = 0; k < 8; ++k) {
+        T s = 0;
+        for (int v = 0; v < nv; ++v)
+            if (wv[v] > T(0)) {
+                const T x = ca[size_t(k) * nv + v];
+                if (gt && flag[k])
+                    s += wv[v] * (x + T(0.1) * gt[size_t(k) * nv + v]);
+                else
+                    s += wv[v] * x;
+            }
+        fc[k] = s / wn;
+    }
+}
+
+// Adoption of the cheapest candidate (lib 1306-1334, src 1023-1039): the
+// last index of the minimum final cost; adopted only if that candidate is
+// flagged, its depth dk lies in [dmin, dmax] and it is strictly cheaper than
+// the current plane. Returns the candidate or -1.
+template <class T>
+int adopt_min(const T fc[8], const bool flag[8], const T dk[8], T cnow, T dmin, T dmax) {
+    const int k = argmin_last(fc, 8);
+    if (!flag[k]) return -1;
+    return (dk[k] >= dmin && dk[k] <= dmax && fc[k] < cnow) ? k : -1;
+}
+
+// Adoption under the planar prior (lib 1278-1305): the last index of the
+// maximum restricted score rs (unflagged candidates score 0); adopted only if
+// flagged, in range and strictly above the current plane's score rnow.
+template <class T>
+int adopt_max(const T rs[8], const bool flag[8], const T dk[8], T rnow, T dmin, T dmax) {
+    const int k = argmax_last(rs, 8);
+    if (!flag[k]) return -1;
+    return (dk[k] >= dmin && dk[k] <= dmax && rs[k] > rnow) ? k : -1;
+}
+
+// The five refinement hypotheses in the reference's order (lib 740-749):
+// depths {d_rand, d_cur, d_rand, d_cur, d_pert} with normals {n_cur, n_rand,
+// n_rand, n_pert, n_cur}; n_cur and d_cur are the plane after the
+// propagation step (captured once, before any refinement adoption).
+template <class T>
+void refine_hypotheses(T drand, const V4<T>& nrand, T dcur, const V4<T>& ncur, T dpert,
+                       const V4<T>& npert, T depths[5], V4<T> normals[5]) {
+    depths[0] = drand;
+    depths[1] = dcur;
+    depths[2] = drand;
+    depths[3] = dcur;
+    depths[4] = dpert;
+    normals[0] = ncur;
+    normals[1] = nrand;
+    normals[2] = nrand;
+    normals[3] = npert;
+    normals[4] = ncur;
+}
+
+// Sequential adoption of the refinement hypotheses (lib 751-810): each
+// hypothesis i with cost tc[i] and depth db[i] is compared with the state as
+// left by the hypotheses before it. Plain: adopt if db in [dmin, dmax] and
+// tc < *cost. Prior (pm): adopt if in range and its score sc[i] > *restricted
+// (the restricted score is 0 when the propagation step adopted nothing under
+// the prior, lib 1240). Updates *cost (and *restricted) and returns the last
+// adopted hypothesis or -1.
+template <class T>
+int refine_adopt(bool pm, const T tc[5], const T db[5], const T sc[5], T dmin, T dmax, T* cost,
+                 T* restricted) {
+    int last = -1;
+    for (int i = 0; i < 5; ++i) {
+        const bool in = db[i] >= dmin && db[i] <= dmax;
+        if (pm) {
+            if (in && sc[i] > *restricted) {
+                *cost = tc[i];
+                *restricted = sc[i];
+                last = i;
+            }
+        } else if (in && tc[i] < *cost) {
+            *cost = tc[i];
+            last = i;
+        }
+    }
+    return last;
+}
+
+// Restricted (planar-prior) score (lib 1268-1277, 784-793):
+// exp(-c^2 / beta) * (gamma + exp(-dd^2 / 2 sd^2) exp(-a^2 / 2 sa^2)) with
+// gamma = 0.5, beta = 0.18, sd = (dmax - dmin) / 64, sa = 5 degrees,
+// dd = depth - depth_prior, a = acos(n . n_prior).
+template <class T>
+T prior_score(T cost_v, T depth_v, T depth_prior, const V4<T>& pp, const V4<T>& pl, T dmin,
+              T dmax) {
+    const T gamma = T(0.5), beta = T(0.18);
+    const T dsig = (dmax - dmin) / T(64);
+    const T two_ds2 = T(2) * dsig * dsig;
+    const T asig = T(kPi) * (T(5) / T(180));
+    const T two_as2 = T(2) * asig * asig;
+    const T dd = depth_v - depth_prior;
+    // The dot product of two unit normals is clamped into [-1, 1] (NaN
+    // stays NaN): the reference's acos of the raw dot (lib 1266, 1276,
+    // 789) returns NaN for identical normals whenever rounding lands just
+    // above 1, which silently drops the candidate in one precision and
+    // not in another.
+    T dot = pp.x * pl.x + pp.y * pl.y + pp.z * pl.z;
+    dot = dot > T(1) ? T(1) : (dot < T(-1) ? T(-1) : dot);
+    const T ang = std::acos(dot);
+    const T pr = gamma + std::exp(-dd * dd / two_ds2) * std::exp(-ang * ang / two_as2);
+    return std::exp(-cost_v * cost_v / beta) * pr;
+}
+
 // --------------------------------------------------------------- the run ---
 template <class T>
 struct Run {
@@ -497,192 +750,9 @@ struct Run {
         sel[idx] = bits;
     }
 
-    // Candidate search of checkerboard_propagation (src 720-913, lib 862-1104)
-    // against the snapshot. flag[k] and pos[k] use the reference's order:
-    // 0 up_near, 1 up_far, 2 down_near, 3 down_far, 4 left_near, 5 left_far,
-    // 6 right_near, 7 right_far.
-    void candidates(int px, int py, bool flag[8], int pos[8]) const {
-        const int c = py * w + px;
-        for (int k = 0; k < 8; ++k) {
-            flag[k] = false;
-            pos[k] = -1;
-        }
-        const std::vector<T>& cs = cost0;
-        if (py > 2) {
-            flag[1] = true;
-            int best = c - 3 * w;
-            T m = cs[best];
-            for (int i = 1; i < 11; ++i)
-                if (py > 2 + 2 * i) {
-                    const int q = c - 3 * w - 2 * i * w;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            pos[1] = best;
-        }
-        if (py < h - 3) {
-            flag[3] = true;
-            int best = c + 3 * w;
-            T m = cs[best];
-            for (int i = 1; i < 11; ++i)
-                if (py < h - 3 - 2 * i) {
-                    const int q = c + 3 * w + 2 * i * w;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            pos[3] = best;
-        }
-        if (px > 2) {
-            flag[5] = true;
-            int best = c - 3;
-            T m = cs[best];
-            for (int i = 1; i < 11; ++i)
-                if (px > 2 + 2 * i) {
-                    const int q = c - 3 - 2 * i;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            pos[5] = best;
-        }
-        if (px < w - 3) {
-            flag[7] = true;
-            int best = c + 3;
-            T m = cs[best];
-            for (int i = 1; i < 11; ++i)
-                if (px < w - 3 - 2 * i) {
-                    const int q = c + 3 + 2 * i;
-                    // keeps the maximum, as the reference does, unless
-                    // OCOMPAT_FIX_RIGHT_FAR
-                    const bool better = (a.compat & OCOMPAT_FIX_RIGHT_FAR) ? (cs[q] < m) : (m < cs[q]);
-                    if (better) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            pos[7] = best;
-        }
-        if (py > 0) {
-            flag[0] = true;
-            const int base = c - w;
-            int best = base;
-            T m = cs[best];
-            for (int i = 0; i < 3; ++i) {
-                if (py > 1 + i && px > i) {
-                    const int q = base - (1 + i) * w - i;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-                if (py > 1 + i && px < w - 1 - i) {
-                    const int q = base - (1 + i) * w + i;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            }
-            pos[0] = best;
-        }
-        if (py < h - 1) {
-            flag[2] = true;
-            const int base = c + w;
-            int best = base;
-            T m = cs[best];
-            for (int i = 0; i < 3; ++i) {
-                if (py < h - 2 - i && px > i) {
-                    const int q = base + (1 + i) * w - i;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-                if (py < h - 2 - i && px < w - 1 - i) {
-                    const int q = base + (1 + i) * w + i;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            }
-            pos[2] = best;
-        }
-        if (px > 0) {
-            flag[4] = true;
-            const int base = c - 1;
-            int best = base;
-            T m = cs[best];
-            for (int i = 0; i < 3; ++i) {
-                if (px > 1 + i && py > i) {
-                    const int q = base - (1 + i) - i * w;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-                if (px > 1 + i && py < h - 1 - i) {
-                    const int q = base - (1 + i) + i * w;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            }
-            pos[4] = best;
-        }
-        if (px < w - 1) {
-            flag[6] = true;
-            const int base = c + 1;
-            int best = base;
-            T m = cs[best];
-            for (int i = 0; i < 3; ++i) {
-                if (px < w - 2 - i && py > i) {
-                    const int q = base + (1 + i) - i * w;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-                if (px < w - 2 - i && py < h - 1 - i) {
-                    const int q = base + (1 + i) + i * w;
-                    if (cs[q] < m) {
-                        m = cs[q];
-                        best = q;
-                    }
-                }
-            }
-            pos[6] = best;
-        }
-    }
-
-    // Restricted (planar-prior) score (lib 1268-1277, 784-793).
-    T prior_score(T cost_v, T depth_v, T depth_prior, const V4<T>& pp, const V4<T>& pl) const {
-        const T gamma = T(0.5), beta = T(0.18);
-        const T dsig = (dmax - dmin) / T(64);
-        const T two_ds2 = T(2) * dsig * dsig;
-        const T asig = T(kPi) * (T(5) / T(180));
-        const T two_as2 = T(2) * asig * asig;
-        const T dd = depth_v - depth_prior;
-        // The dot product of two unit normals is clamped into [-1, 1] (NaN
-        // stays NaN): the reference's acos of the raw dot (lib 1266, 1276,
-        // 789) returns NaN for identical normals whenever rounding lands just
-        // above 1, which silently drops the candidate in one precision and
-        // not in another.
-        T dot = pp.x * pl.x + pp.y * pl.y + pp.z * pl.z;
-        dot = dot > T(1) ? T(1) : (dot < T(-1) ? T(-1) : dot);
-        const T ang = std::acos(dot);
-        const T pr = gamma + std::exp(-dd * dd / two_ds2) * std::exp(-ang * ang / two_as2);
-        return std::exp(-cost_v * cost_v / beta) * pr;
-    }
-
     // One pixel of one half-sweep: propagation (src 678-1041, lib 813-1351)
-    // then refinement (src 595-676, lib 684-811).
+    // then refinement (src 595-676, lib 684-811), as the decision steps above
+    // applied to this pixel's NCCs.
     void update_pixel(int px, int py, int iter) {
         const int c = py * w + px;
         const Cam<T>& rc = cams[0];
@@ -692,7 +762,7 @@ struct Run {
 
         bool flag[8];
         int pos[8];
-        candidates(px, py, flag, pos);
+        candidate_search<T>(cost0.data(), h, w, px, py, a.compat, flag, pos);
 
         // cost_array[8][32] = {2.0f}: only [0][0] is 2, the rest are 0
         // (all 2 with OCOMPAT_FIX_COST_DEFAULT).
@@ -702,77 +772,29 @@ struct Run {
             if (flag[k])
                 for (int v = 0; v < nv; ++v) ca[size_t(k) * nv + v] = ncc(v, px, py, plane0[pos[k]]);
 
-        // Joint view selection (src 915-981, lib 1114-1183).
-        std::vector<T> prior_v(nv, T(0));
         const int nb[4] = {c - w, c + w, c - 1, c + 1};
-        for (int i = 0; i < 4; ++i)
-            if (flag[2 * i])
-                for (int v = 0; v < nv; ++v)
-                    prior_v[v] += ((sel[nb[i]] >> v) & 1u) ? T(0.9) : T(0.1);
-        std::vector<T> prob(nv, T(0));
-        const T tau = T(0.8) * std::exp(T(iter) * T(iter) / T(-90));
-        for (int v = 0; v < nv; ++v) {
-            T count = 0, tmpw = 0;
-            int cf = 0;
-            for (int k = 0; k < 8; ++k) {
-                const T x = ca[size_t(k) * nv + v];
-                if (x < tau) {
-                    tmpw += std::exp(x * x / T(-0.18));
-                    count += T(1);
-                }
-                if (x > T(1.2)) ++cf;
-            }
-            if (count > T(2) && cf < 3)
-                prob[v] = tmpw / count;
-            else if (cf < 3)
-                prob[v] = std::exp(tau * tau / T(-0.32));
-            prob[v] = prob[v] * prior_v[v];
-        }
-        {  // transform_PDF_to_CDF (src 79-92)
-            T sum = 0;
-            for (int v = 0; v < nv; ++v) sum += prob[v];
-            const T inv = T(1) / sum;
-            T cum = 0;
-            for (int v = 0; v < nv; ++v) {
-                cum += prob[v] * inv;
-                prob[v] = cum;
-            }
-        }
-        std::vector<T> wv(nv, T(0));
-        for (int d = 0; d < 15; ++d) {
-            const T rp = T(r.u(ORNG_VIEW + d)) - T(FLT_EPSILON);
-            for (int v = 0; v < nv; ++v)
-                if (prob[v] > rp) {
-                    wv[v] += T(1);
-                    break;
-                }
-        }
-        uint32_t tsel = 0;
-        T wn = 0;
-        for (int v = 0; v < nv; ++v)
-            if (wv[v] > T(0)) {
-                tsel |= 1u << v;
-                wn += wv[v];
-            }
+        uint32_t nbsel[4];
+        for (int i = 0; i < 4; ++i) nbsel[i] = flag[2 * i] ? sel[nb[i]] : 0u;
+        T draws[15];
+        for (int d = 0; d < 15; ++d) draws[d] = T(r.u(ORNG_VIEW + d));
+        T cdf[32], wv[32], wn;
+        uint32_t tsel;
+        select_views<T>(ca.data(), nv, flag, nbsel, iter, draws, cdf, wv, &tsel, &wn);
 
-        T fc[8];
-        for (int k = 0; k < 8; ++k) {
-            T s = 0;
-            for (int v = 0; v < nv; ++v)
-                if (wv[v] > T(0)) {
-                    const T x = ca[size_t(k) * nv + v];
-                    if (geo && flag[k])
-                        s += wv[v] * (x + T(0.1) * geom(v, px, py, plane0[pos[k]]));
-                    else
-                        s += wv[v] * x;
-                }
-            fc[k] = s / wn;
+        std::vector<T> gt;
+        if (geo) {
+            gt.assign(size_t(8) * nv, T(0));
+            for (int k = 0; k < 8; ++k)
+                if (flag[k])
+                    for (int v = 0; v < nv; ++v)
+                        if (wv[v] > T(0)) gt[size_t(k) * nv + v] = geom(v, px, py, plane0[pos[k]]);
         }
-        const int kmin = argmin_last(fc, 8);
+        T fc[8];
+        final_costs<T>(ca.data(), nv, flag, wv, wn, geo ? gt.data() : nullptr, fc);
 
         // Current hypothesis with the new view weights; views with zero
         // weight add exactly zero and are skipped.
-        V4<T> cur = plane[c];
+        const V4<T> cur = plane[c];
         T cnow = 0;
         for (int v = 0; v < nv; ++v)
             if (wv[v] > T(0)) {
@@ -783,65 +805,48 @@ struct Run {
         cost[c] = cnow;
         T depth_now = depth_from_plane(rc, cur, px, py);
 
+        T dk[8];
+        for (int k = 0; k < 8; ++k)
+            dk[k] = flag[k] ? depth_from_plane(rc, plane0[pos[k]], px, py) : T(0);
+        const bool pm = prior && masked(c);
+        T dprior = T(0);
+        V4<T> pp{0, 0, 0, 0};
+        if (prior && a.prior_planes) {
+            pp = prior_plane(c);
+            dprior = depth_from_plane(rc, pp, px, py);
+        }
         T restricted = T(0);
-        if (prior) {
-            T dprior = T(0);
-            V4<T> pp{0, 0, 0, 0};
-            if (a.prior_planes) {
-                pp = prior_plane(c);
-                dprior = depth_from_plane(rc, pp, px, py);
-            }
-            if (masked(c)) {
-                T rs[8];
-                for (int k = 0; k < 8; ++k) {
-                    rs[k] = T(0);
-                    if (flag[k]) {
-                        const T dk = depth_from_plane(rc, plane0[pos[k]], px, py);
-                        rs[k] = prior_score(fc[k], dk, dprior, pp, plane0[pos[k]]);
-                    }
-                }
-                const int kmax = argmax_last(rs, 8);
-                const T rnow = prior_score(cnow, depth_from_plane(rc, cur, px, py), dprior, pp, cur);
-                if (flag[kmax]) {
-                    const T db = depth_from_plane(rc, plane0[pos[kmax]], px, py);
-                    if (db >= dmin && db <= dmax && rs[kmax] > rnow) {
-                        depth_now = db;
-                        plane[c] = plane0[pos[kmax]];
-                        cost[c] = fc[kmax];
-                        restricted = rs[kmax];
-                        sel[c] = tsel;
-                    }
-                }
-            } else if (flag[kmin]) {
-                const T db = depth_from_plane(rc, plane0[pos[kmin]], px, py);
-                if (db >= dmin && db <= dmax && fc[kmin] < cnow) {
-                    depth_now = db;
-                    plane[c] = plane0[pos[kmin]];
-                    cost[c] = fc[kmin];
-                    if (a.compat & OCOMPAT_FIX_PRIOR_SELECTION) sel[c] = tsel;
-                }
-            }
-        } else if (flag[kmin]) {
-            const T db = depth_from_plane(rc, plane0[pos[kmin]], px, py);
-            if (db >= dmin && db <= dmax && fc[kmin] < cnow) {
-                depth_now = db;
-                plane[c] = plane0[pos[kmin]];
-                cost[c] = fc[kmin];
+        if (pm) {
+            T rs[8];
+            for (int k = 0; k < 8; ++k)
+                rs[k] = flag[k] ? prior_score(fc[k], dk[k], dprior, pp, plane0[pos[k]], dmin, dmax)
+                                : T(0);
+            const T rnow = prior_score(cnow, depth_now, dprior, pp, cur, dmin, dmax);
+            const int k = adopt_max<T>(rs, flag, dk, rnow, dmin, dmax);
+            if (k >= 0) {
+                depth_now = dk[k];
+                plane[c] = plane0[pos[k]];
+                cost[c] = fc[k];
+                restricted = rs[k];
                 sel[c] = tsel;
+            }
+        } else {
+            const int k = adopt_min<T>(fc, flag, dk, cnow, dmin, dmax);
+            if (k >= 0) {
+                depth_now = dk[k];
+                plane[c] = plane0[pos[k]];
+                cost[c] = fc[k];
+                // the prior path leaves selected_views alone (lib 1306-1318)
+                if (!prior || (a.compat & OCOMPAT_FIX_PRIOR_SELECTION)) sel[c] = tsel;
             }
         }
 
         // ---- refinement
         const T pert = T(0.02);
-        const bool pm = prior && masked(c);
-        T dprior = T(0);
-        V4<T> pp{0, 0, 0, 0};
         T drand;
         V4<T> nrand;
         const T dsig = (dmax - dmin) / T(64);
         if (pm) {
-            pp = prior_plane(c);
-            dprior = depth_from_plane(rc, pp, px, py);
             drand = T(r.u(ORNG_DEPTH_RAND)) * T(6) * dsig + (dprior - T(3) * dsig);
             nrand = perturbed_normal(rc, px, py, pp, r, ORNG_PRIOR_ANGLE,
                                      T(kPi) * (T(5) / T(180)));
@@ -849,38 +854,35 @@ struct Run {
             drand = T(r.u(ORNG_DEPTH_RAND)) * (dmax - dmin) + dmin;
             nrand = random_normal(rc, px, py, r, ORNG_NORMAL, depth_now);
         }
+        // the reference's do { draw } while (d < dmin && d > dmax) runs once
         const T dlo = (T(1) - pert) * depth_now;
         const T dhi = (T(1) + pert) * depth_now;
         const T dpert = T(r.u(ORNG_DEPTH_PERT)) * (dhi - dlo) + dlo;
         const V4<T> ncur = plane[c];
         const V4<T> npert = perturbed_normal(rc, px, py, ncur, r, ORNG_PERT_ANGLE, pert * T(kPi));
-        const T dcur = depth_now;
-        const T depths[5] = {drand, dcur, drand, dcur, dpert};
-        const V4<T> normals[5] = {ncur, nrand, nrand, npert, ncur};
+        T depths[5];
+        V4<T> normals[5];
+        refine_hypotheses<T>(drand, nrand, depth_now, ncur, dpert, npert, depths, normals);
+        V4<T> tp[5];
+        T tc[5], db[5], sc[5];
         for (int i = 0; i < 5; ++i) {
-            V4<T> tp = normals[i];
-            tp.w = dist_to_origin(rc, px, py, depths[i], tp);
-            T tc = 0;
+            tp[i] = normals[i];
+            tp[i].w = dist_to_origin(rc, px, py, depths[i], tp[i]);
+            T s = 0;
             for (int v = 0; v < nv; ++v)
                 if (wv[v] > T(0)) {
-                    const T x = ncc(v, px, py, tp);
-                    tc += wv[v] * (geo ? x + T(0.1) * geom(v, px, py, tp) : x);
+                    const T x = ncc(v, px, py, tp[i]);
+                    s += wv[v] * (geo ? x + T(0.1) * geom(v, px, py, tp[i]) : x);
                 }
-            tc /= wn;
-            const T db = depth_from_plane(rc, tp, px, py);
-            if (pm) {
-                const T sc = prior_score(tc, depths[i], dprior, pp, tp);
-                if (db >= dmin && db <= dmax && sc > restricted) {
-                    depth_now = db;
-                    plane[c] = tp;
-                    cost[c] = tc;
-                    restricted = sc;
-                }
-            } else if (db >= dmin && db <= dmax && tc < cost[c]) {
-                depth_now = db;
-                plane[c] = tp;
-                cost[c] = tc;
-            }
+            tc[i] = s / wn;
+            db[i] = depth_from_plane(rc, tp[i], px, py);
+            sc[i] = pm ? prior_score(tc[i], depths[i], dprior, pp, tp[i], dmin, dmax) : T(0);
+        }
+        T cst = cost[c];
+        const int k = refine_adopt<T>(pm, tc, db, sc, dmin, dmax, &cst, &restricted);
+        if (k >= 0) {
+            plane[c] = tp[k];
+            cost[c] = cst;
         }
     }
 
@@ -1041,6 +1043,85 @@ int oracle_geom_batch(const OCamera* cams, int num_images, const float* const* d
                 geom_cost<double>(depths[v + 1], c[0], c[v + 1], pl, xy[2 * q], xy[2 * q + 1]);
     }
     return 0;
+}
+
+// ---- FP64 hooks of the decision steps, for known-answer tests on hand-built
+// inputs. Each calls the same template the pass uses.
+
+// Candidate search over an h x w cost grid at (px, py): flag[8], pos[8].
+int oracle_t_candidates(const double* costs, int h, int w, int px, int py, uint32_t compat,
+                        int32_t* flag, int32_t* pos) {
+    if (!costs || px < 0 || py < 0 || px >= w || py >= h) return 1;
+    bool f[8];
+    int p[8];
+    candidate_search<double>(costs, h, w, px, py, compat, f, p);
+    for (int k = 0; k < 8; ++k) {
+        flag[k] = f[k] ? 1 : 0;
+        pos[k] = p[k];
+    }
+    return 0;
+}
+
+// View selection from an 8 x nv cost table, the four neighbour masks and 15
+// given draws.
+int oracle_t_select_views(const double* ca, int nv, const int32_t* flag, const uint32_t* nbsel,
+                          int iter, const double* draws, double* cdf, double* wv, uint32_t* tsel,
+                          double* wn) {
+    if (!ca || nv < 1 || nv > 32) return 1;
+    bool f[8];
+    for (int k = 0; k < 8; ++k) f[k] = flag[k] != 0;
+    select_views<double>(ca, nv, f, nbsel, iter, draws, cdf, wv, tsel, wn);
+    return 0;
+}
+
+// Final costs of the 8 candidates; gt (8 x nv geometric terms) may be NULL.
+int oracle_t_final_costs(const double* ca, int nv, const int32_t* flag, const double* wv,
+                         double wn, const double* gt, double* fc) {
+    if (!ca || nv < 1 || nv > 32) return 1;
+    bool f[8];
+    for (int k = 0; k < 8; ++k) f[k] = flag[k] != 0;
+    final_costs<double>(ca, nv, f, wv, wn, gt, fc);
+    return 0;
+}
+
+// Propagation adoption: prior = 0 takes final costs and the current cost
+// (adopt_min), prior = 1 restricted scores and the current score
+// (adopt_max). Returns the adopted candidate or -1.
+int oracle_t_adopt(int prior, const double* score, const int32_t* flag, const double* dk,
+                   double now, double dmin, double dmax) {
+    bool f[8];
+    for (int k = 0; k < 8; ++k) f[k] = flag[k] != 0;
+    return prior ? adopt_max<double>(score, f, dk, now, dmin, dmax)
+                 : adopt_min<double>(score, f, dk, now, dmin, dmax);
+}
+
+// The five refinement hypotheses: depths[5], normals[5 * 3].
+void oracle_t_refine_hypotheses(double drand, const double* nrand, double dcur,
+                                const double* ncur, double dpert, const double* npert,
+                                double* depths, double* normals) {
+    const V4<double> nr{nrand[0], nrand[1], nrand[2], 0};
+    const V4<double> nc{ncur[0], ncur[1], ncur[2], 0};
+    const V4<double> np{npert[0], npert[1], npert[2], 0};
+    V4<double> n[5];
+    refine_hypotheses<double>(drand, nr, dcur, nc, dpert, np, depths, n);
+    for (int i = 0; i < 5; ++i) {
+        normals[3 * i] = n[i].x;
+        normals[3 * i + 1] = n[i].y;
+        normals[3 * i + 2] = n[i].z;
+    }
+}
+
+// Sequential refinement adoption; *cost and *restricted are in/out.
+int oracle_t_refine_adopt(int pm, const double* tc, const double* db, const double* sc,
+                          double dmin, double dmax, double* cost, double* restricted) {
+    return refine_adopt<double>(pm != 0, tc, db, sc, dmin, dmax, cost, restricted);
+}
+
+// Restricted score of one (cost, depth, normal) against a prior plane.
+double oracle_t_prior_score(double cost, double depth, double depth_prior, const double* pp,
+                            const double* pl, double dmin, double dmax) {
+    const V4<double> a{pp[0], pp[1], pp[2], 0}, b{pl[0], pl[1], pl[2], 0};
+    return prior_score<double>(cost, depth, depth_prior, a, b, dmin, dmax);
 }
 
 // One full PatchMatch pass; precision 64 = oracle, 32 = CPU baseline.

@@ -1,0 +1,161 @@
+"""Closed-form tests of the sequential FP32 fusion twin (oracle/fusion_oracle.cpp).
+
+The GPU fusion kernel is compared with this twin bit for bit; these cases pin
+the twin itself against batch_fusion_kernel (gmvs/src/fusion.cu:61-182) with
+cameras and depths chosen so every intermediate value is a short binary
+fraction (exact in FP32): consistency, the count + 1 average, the rounding
+int(x + 0.5f) with truncation toward zero, the strict threshold tests, the
+last writer of a scatter collision and the mask side effects across views.
+"""
+import numpy as np
+
+from oracle import oracle as O
+
+H, W = 4, 8
+
+
+def _cam(fx, fy=128.0, cx=0.0, cy=0.0, t=(0.0, 0.0, 0.0)):
+    K = np.array([[fx, 0, cx], [0, fy, cy], [0, 0, 1.0]])
+    return O.make_camera(K, np.eye(3), t, H, W, 0.5, 10.0)
+
+
+def _const(v):
+    return np.full((H, W), v, np.float32)
+
+
+def _normals(fn):
+    n = np.zeros((H, W, 3), np.float32)
+    for r in range(H):
+        for c in range(W):
+            n[r, c] = fn(c, r)
+    return n
+
+
+NZ = _normals(lambda c, r: (0.0, 0.0, -1.0))
+
+
+def _fuse(cams, depths, normals, masks, ref_id, src_ids, geom=1, reproj=2.0, rel=0.01,
+          cos_angle=-2.0):
+    return O.fuse_view(cams, depths, normals, masks, ref_id, src_ids, geom, reproj, rel, cos_angle)
+
+
+def _masks(n=3):
+    return np.zeros((n, H, W), np.uint8)
+
+
+def test_identical_views_every_pixel_consistent():
+    cam = _cam(128.0)
+    m = _masks()
+    d, n, cnt = _fuse([cam, cam], [_const(2.0), _const(2.0)], [NZ, NZ], m, 0, [1])
+    assert cnt == H * W
+    assert np.array_equal(d, _const(2.0))        # (X + X) / 2 = X, back on its own pixel
+    assert np.array_equal(n, NZ)
+    assert m[1].all() and not m[0].any() and not m[2].any()
+
+
+def test_outlier_sources_give_nothing():
+    cam = _cam(128.0)
+    m = _masks()
+    d, n, cnt = _fuse([cam, cam, cam], [_const(2.0), _const(4.0), _const(1.0)], [NZ, NZ, NZ], m, 0,
+                      [1, 2])
+    assert cnt == 0 and not d.any() and not n.any() and not m.any()
+
+
+def test_geom_consistent_is_a_minimum_count():
+    cam = _cam(128.0)
+    args = ([cam, cam, cam], [_const(2.0)] * 3, [NZ] * 3)
+    assert _fuse(*args, _masks(), 0, [1, 2], geom=2)[2] == H * W
+    assert _fuse(*args, _masks(), 0, [1, 2], geom=3)[2] == 0
+
+
+def _collision_case():
+    """Reference fx = 128, source fx = 64, both at depth 2. Reference column
+    c projects to source column c / 2, which int(x + 0.5f) rounds up for odd
+    c; that source pixel reprojects to 2 * ceil(c / 2): error 0 (even c) or 1
+    (odd c). The fused point, the mean of the two, projects to column c (even)
+    or c + 1.5 -> c + 1 (odd): odd column c and even column c + 1 land on
+    the same output pixel."""
+    ref, src = _cam(128.0), _cam(64.0)
+    nr = _normals(lambda c, r: (c / 8.0, r / 8.0, -1.0))
+    ns = _normals(lambda c, r: (0.5, 0.25, -c / 16.0))
+    return [ref, src], [_const(2.0), _const(2.0)], [nr, ns]
+
+
+def test_collision_highest_reference_pixel_wins():
+    cams, depths, normals = _collision_case()
+    m = _masks(2)
+    d, n, cnt = _fuse(cams, depths, normals, m, 0, [1], reproj=2.0)
+    assert cnt == H * W
+    nr, ns = normals
+    for r in range(H):
+        for c in range(W):
+            if c % 2:
+                assert d[r, c] == 0 and not n[r, c].any()
+            else:
+                # written by columns c - 1 and c: the higher index (c) wins;
+                # its source pixel is column c / 2; the normal is averaged over
+                # count + 1 = 2 and not normalised
+                assert d[r, c] == 2.0
+                assert np.array_equal(n[r, c], (nr[r, c] + ns[r, c // 2]) / 2)
+    # source columns used: ceil(c / 2) for c = 0..7 -> 0..4
+    assert m[1][:, :5].all() and not m[1][:, 5:].any() and not m[0].any()
+
+
+def test_reprojection_threshold_is_strict():
+    cams, depths, normals = _collision_case()
+    # odd columns reproject with an error of exactly 1 px
+    _, _, cnt = _fuse(cams, depths, normals, _masks(2), 0, [1], reproj=1.0)
+    assert cnt == H * W // 2
+    _, _, cnt = _fuse(cams, depths, normals, _masks(2), 0, [1], reproj=1.0000001)
+    assert cnt == H * W
+
+
+def test_relative_depth_threshold_is_strict():
+    cam = _cam(128.0)
+    # |3 - 2| / 2 = 0.5 exactly
+    args = ([cam, cam], [_const(2.0), _const(3.0)], [NZ, NZ])
+    assert _fuse(*args, _masks(2), 0, [1], rel=0.5)[2] == 0
+    assert _fuse(*args, _masks(2), 0, [1], rel=0.5000001)[2] == H * W
+
+
+def test_normal_angle_test():
+    cam = _cam(128.0)
+    tilt = _normals(lambda c, r: (0.6, 0.0, -0.8))
+    args = ([cam, cam], [_const(2.0), _const(2.0)])
+    # dot = 0.8: passes cos(theta) = 0.7, fails 0.8 (the angle must be
+    # strictly smaller than theta)
+    assert _fuse(*args, [NZ, tilt], _masks(2), 0, [1], cos_angle=0.7)[2] == H * W
+    assert _fuse(*args, [NZ, tilt], _masks(2), 0, [1], cos_angle=np.float32(0.8))[2] == 0
+    # a dot product outside [-1, 1] passes, as acosf's NaN does in get_angle_cu
+    # (fusion.cu:50-58), also one of -2 (normals pointing apart)
+    for v in (-2.0, 2.0, np.nan):
+        odd = _normals(lambda c, r: (0.0, 0.0, v))
+        assert _fuse(*args, [NZ, odd], _masks(2), 0, [1], cos_angle=0.99)[2] == H * W
+
+
+def test_rounding_truncates_toward_zero():
+    """Source principal point cx = -0.75: reference column c projects to
+    source x = c - 0.75. For c = 0, x + 0.5 = -0.25 truncates to column 0 and
+    is accepted (fusion.cu:112-113); with cx = -1.5, x + 0.5 = -1 for c = 0
+    and the projection falls outside."""
+    ref = _cam(128.0)
+    args = ([_const(2.0), _const(2.0)], [NZ, NZ])
+    d, _, cnt = _fuse([ref, _cam(128.0, cx=-0.75)], *args, _masks(2), 0, [1], reproj=1.0)
+    assert cnt == H * W  # errors 0.75 (column 0) and 0.25
+    _, _, cnt = _fuse([ref, _cam(128.0, cx=-1.5)], *args, _masks(2), 0, [1], reproj=2.0)
+    assert cnt == H * (W - 1)  # column 0 finds no source pixel
+
+
+def test_masks_carry_across_views():
+    cam = _cam(128.0)
+    cams = [cam, cam]
+    depths = [_const(2.0), _const(2.0)]
+    m = _masks(2)
+    _, _, c0 = _fuse(cams, depths, [NZ, NZ], m, 0, [1])
+    assert c0 == H * W and m[1].all() and not m[0].any()
+    # view 1 is now masked everywhere: nothing to fuse
+    d1, _, c1 = _fuse(cams, depths, [NZ, NZ], m, 1, [0])
+    assert c1 == 0 and not d1.any() and not m[0].any()
+    # in the other order view 1 fuses everything
+    m2 = _masks(2)
+    assert _fuse(cams, depths, [NZ, NZ], m2, 1, [0])[2] == H * W and m2[0].all()

@@ -82,6 +82,8 @@ def _load():
     lib.oracle_geom_batch.argtypes = [ctypes.POINTER(OCamera), ctypes.c_int, ctypes.POINTER(vp),
                                       ctypes.c_int, vp, vp, vp]
     lib.oracle_pm_run.argtypes = [ctypes.c_int, ctypes.POINTER(OPMArgs)]
+    lib.oracle_pm_lockstep.argtypes = [ctypes.POINTER(OPMArgs), ctypes.c_int32, vp, vp, vp, vp]
+    lib.oracle_pm_lockstep.restype = ctypes.c_int32
     lib.oracle_max_threads.restype = ctypes.c_int
     lib.oracle_fuse_view.argtypes = [ctypes.POINTER(OCamera), ctypes.c_int, ctypes.POINTER(vp),
                                      ctypes.POINTER(vp), vp, ctypes.c_int, vp, ctypes.c_int,
@@ -204,7 +206,7 @@ def pm_run(cams: Sequence[OCamera], images: Sequence[np.ndarray], params: OParam
            ref_normal: Optional[np.ndarray] = None, prior_planes: Optional[np.ndarray] = None,
            plane_masks: Optional[np.ndarray] = None, planes: Optional[np.ndarray] = None,
            costs: Optional[np.ndarray] = None, threads: int = 0, skip_filter: bool = False,
-           compat: int = 0):
+           compat: int = 0, lockstep: int = 0):
     """One PatchMatch pass. Returns dict(depth, normal, cost, planes, selected_views)."""
     imgs = [_f32(a) for a in images]
     h, w = imgs[0].shape
@@ -245,10 +247,33 @@ def pm_run(cams: Sequence[OCamera], images: Sequence[np.ndarray], params: OParam
     a.out_depth = _ptr(od)
     a.out_normal = _ptr(on)
     a.out_cost = _ptr(oc)
+    keep = (cam_arr, img_ptrs, dep_ptrs, imgs, ds, rn, pp, pmk)  # alive until the call returns
+    if lockstep:
+        cap = lockstep
+        ids = np.zeros((cap, 5), np.int32)
+        mg = np.zeros((cap, 4), np.float64)
+        cnt = np.zeros(1, np.int64)
+        vals = np.zeros((cap, 35), np.float64)
+        rows = lib().oracle_pm_lockstep(ctypes.byref(a), cap, _ptr(ids), _ptr(mg), _ptr(cnt),
+                                        _ptr(vals))
+        assert rows >= 0, f"oracle_pm_lockstep returned {rows}"
+        del keep
+        return dict(ids=ids[:rows], margins=mg[:rows], decisions=int(cnt[0]), values=vals[:rows])
     rc = lib().oracle_pm_run(precision, ctypes.byref(a))
+    del keep
     assert rc == 0, f"oracle_pm_run returned {rc}"
     return dict(depth=od, normal=on, cost=oc, planes=st_planes.reshape(h, w, 4),
                 selected_views=sel.reshape(h, w))
+
+
+def pm_lockstep(cams, images, params, cap: int = 1 << 20, **kw):
+    """FP32 against FP64 from the same FP64 state at every half-sweep: the
+    pixel updates whose decisions differ, as dict(ids [n, 5] = (iteration,
+    colour, pixel, what, masked), margins [n, 4] = FP64 (draw, argext, adopt,
+    refine) margins, decisions = pixel updates compared). what: 1 drawn views,
+    2 argmin/argmax, 4 propagation adoption, 8 refinement adoption, 16 an NCC
+    at one of its two discontinuities in one precision only."""
+    return pm_run(cams, images, params, lockstep=cap, **kw)
 
 
 def fuse_view(cams: Sequence[OCamera], depths: Sequence[np.ndarray],

@@ -629,6 +629,25 @@ T prior_score(T cost_v, T depth_v, T depth_prior, const V4<T>& pp, const V4<T>& 
     return std::exp(-cost_v * cost_v / beta) * pr;
 }
 
+// Decisions of one pixel update and their margins (distance of the deciding
+// value from the value it was compared with), recorded only by the lockstep
+// comparison of the FP32 and FP64 instantiations.
+struct Decision {
+    uint32_t tsel;    // drawn views
+    int32_t kext;     // argmin (plain) / argmax (prior) candidate
+    int32_t kprop;    // adopted candidate or -1
+    int32_t kref;     // last adopted refinement hypothesis or -1
+    int32_t pm;       // masked pixel under the prior
+    int32_t n2;       // cost-table entries equal to 2 (invalid view or flat patch)
+    double m_draw;    // min of |cdf[v] - (u - FLT_EPSILON)| and of |cost - tau|, |cost - 1.2|
+    double m_ext;     // gap between the best and the runner-up candidate value
+    double m_adopt;   // |best candidate value - current plane's value|
+    double m_refine;  // min over hypotheses of |value - running threshold|
+    double val[8];    // the 8 candidate values compared (final costs or restricted scores)
+    double fc[8];     // the 8 final costs
+    double now;       // the current plane's value
+};
+
 // --------------------------------------------------------------- the run ---
 template <class T>
 struct Run {
@@ -639,6 +658,7 @@ struct Run {
     std::vector<T> cost, cost0;
     std::vector<uint32_t> sel;
     T dmin, dmax;
+    Decision* rec = nullptr;  // per-pixel decision records (lockstep comparison only)
 
     explicit Run(const OPMArgs& args) : a(args) {
         h = a.cameras[0].height;
@@ -780,6 +800,24 @@ struct Run {
         T cdf[32], wv[32], wn;
         uint32_t tsel;
         select_views<T>(ca.data(), nv, flag, nbsel, iter, draws, cdf, wv, &tsel, &wn);
+        Decision* dr = rec ? rec + c : nullptr;
+        if (dr) {
+            dr->tsel = tsel;
+            double m = 1e30;
+            for (int d = 0; d < 15; ++d)
+                for (int v = 0; v < nv; ++v)
+                    m = std::min(m, std::fabs(double(cdf[v]) - (double(draws[d]) - double(FLT_EPSILON))));
+            // and the cost-table entries' distances to the counting thresholds
+            const double tau = double(T(0.8) * std::exp(T(iter) * T(iter) / T(-90)));
+            for (size_t i = 0; i < ca.size(); ++i)
+                m = std::min(m, std::min(std::fabs(double(ca[i]) - tau), std::fabs(double(ca[i]) - 1.2)));
+            dr->m_draw = m;
+            int n2 = 0;
+            for (size_t i = 0; i < ca.size(); ++i) n2 += ca[i] == T(2);
+            dr->n2 = n2;
+            dr->kprop = -1;
+            dr->kref = -1;
+        }
 
         std::vector<T> gt;
         if (geo) {
@@ -815,6 +853,14 @@ struct Run {
             pp = prior_plane(c);
             dprior = depth_from_plane(rc, pp, px, py);
         }
+        // best / runner-up gap of the 8 candidate values
+        auto gap = [](const T* v, int k) {
+            double g = 1e30;
+            for (int i = 0; i < 8; ++i)
+                if (i != k) g = std::min(g, std::fabs(double(v[i]) - double(v[k])));
+            return g;
+        };
+        if (dr) dr->pm = pm ? 1 : 0;
         T restricted = T(0);
         if (pm) {
             T rs[8];
@@ -823,6 +869,17 @@ struct Run {
                                 : T(0);
             const T rnow = prior_score(cnow, depth_now, dprior, pp, cur, dmin, dmax);
             const int k = adopt_max<T>(rs, flag, dk, rnow, dmin, dmax);
+            if (dr) {
+                dr->kext = argmax_last(rs, 8);
+                dr->kprop = k;
+                dr->m_ext = gap(rs, dr->kext);
+                dr->m_adopt = std::fabs(double(rs[dr->kext]) - double(rnow));
+                for (int i = 0; i < 8; ++i) {
+                    dr->val[i] = double(rs[i]);
+                    dr->fc[i] = double(fc[i]);
+                }
+                dr->now = double(rnow);
+            }
             if (k >= 0) {
                 depth_now = dk[k];
                 plane[c] = plane0[pos[k]];
@@ -832,6 +889,14 @@ struct Run {
             }
         } else {
             const int k = adopt_min<T>(fc, flag, dk, cnow, dmin, dmax);
+            if (dr) {
+                dr->kext = argmin_last(fc, 8);
+                dr->kprop = k;
+                dr->m_ext = gap(fc, dr->kext);
+                dr->m_adopt = std::fabs(double(fc[dr->kext]) - double(cnow));
+                for (int i = 0; i < 8; ++i) dr->val[i] = dr->fc[i] = double(fc[i]);
+                dr->now = double(cnow);
+            }
             if (k >= 0) {
                 depth_now = dk[k];
                 plane[c] = plane0[pos[k]];
@@ -879,7 +944,22 @@ struct Run {
             sc[i] = pm ? prior_score(tc[i], depths[i], dprior, pp, tp[i], dmin, dmax) : T(0);
         }
         T cst = cost[c];
+        if (dr) {
+            double m = 1e30;
+            T c2 = cst, r2 = restricted;
+            for (int i = 0; i < 5; ++i) {
+                m = std::min(m, pm ? std::fabs(double(sc[i]) - double(r2))
+                                   : std::fabs(double(tc[i]) - double(c2)));
+                const bool in = db[i] >= dmin && db[i] <= dmax;
+                if (pm ? (in && sc[i] > r2) : (in && tc[i] < c2)) {
+                    c2 = tc[i];
+                    r2 = sc[i];
+                }
+            }
+            dr->m_refine = m;
+        }
         const int k = refine_adopt<T>(pm, tc, db, sc, dmin, dmax, &cst, &restricted);
+        if (dr) dr->kref = k;
         if (k >= 0) {
             plane[c] = tp[k];
             cost[c] = cst;
@@ -1122,6 +1202,103 @@ double oracle_t_prior_score(double cost, double depth, double depth_prior, const
                             const double* pl, double dmin, double dmax) {
     const V4<double> a{pp[0], pp[1], pp[2], 0}, b{pl[0], pl[1], pl[2], 0};
     return prior_score<double>(cost, depth, depth_prior, a, b, dmin, dmax);
+}
+
+// Lockstep comparison of the FP32 and FP64 instantiations (diagnostics for
+// the tests): the FP64 pass runs as usual; before each half-sweep its state
+// is copied into an FP32 run, both update every pixel of the colour from
+// that same state, and each pixel whose decisions differ is recorded:
+// rows of (iteration, colour, pixel, what, masked) in ids[5 * n] with what =
+// 1 (drawn views differ) | 2 (argmin / argmax differs) | 4 (propagation
+// adoption differs) | 8 (refinement adoption differs) | 16 (an NCC of the
+// cost table is 2 in one precision only: a variance at the 1e-5 floor or a
+// warped centre on the image border), and the FP64 run's
+// margins (m_draw, m_ext, m_adopt, m_refine) in margins[4 * n]. Only the
+// first divergence of each kind in the pixel's update counts: a pixel whose
+// drawn views differ has different costs after that. Returns the number of
+// rows (at most cap) or a negative status; *decisions counts pixel updates.
+int oracle_pm_lockstep(const OPMArgs* args, int32_t cap, int32_t* ids, double* margins,
+                       int64_t* decisions, double* values) {
+    const int rc = check_args(args);
+    if (rc) return -rc;
+    Run<double> rd(*args);
+    Run<float> rf(*args);
+    const int h = rd.h, w = rd.w;
+    const size_t n = size_t(h) * w;
+#pragma omp parallel for
+    for (int py = 0; py < h; ++py)
+        for (int px = 0; px < w; ++px) rd.init_pixel(px, py);
+    std::vector<Decision> recd(n), recf(n);
+    rd.rec = recd.data();
+    rf.rec = recf.data();
+    int32_t rows = 0;
+    int64_t count = 0;
+    for (int it = 0; it < args->params.max_iterations; ++it)
+        for (int color = 0; color < 2; ++color) {
+            rd.plane0 = rd.plane;
+            rd.cost0 = rd.cost;
+            for (size_t i = 0; i < n; ++i) {
+                const V4<double>& p = rd.plane[i];
+                rf.plane[i] = V4<float>{float(p.x), float(p.y), float(p.z), float(p.w)};
+                rf.cost[i] = float(rd.cost[i]);
+                rf.sel[i] = rd.sel[i];
+            }
+            rf.plane0 = rf.plane;
+            rf.cost0 = rf.cost;
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int py = 0; py < h; ++py)
+                for (int px = (py + color) & 1; px < w; px += 2) {
+                    rd.update_pixel(px, py, it);
+                    rf.update_pixel(px, py, it);
+                }
+            for (int py = 0; py < h; ++py)
+                for (int px = (py + color) & 1; px < w; px += 2) {
+                    const size_t c = size_t(py) * w + px;
+                    ++count;
+                    const Decision& a = recd[c];
+                    const Decision& b = recf[c];
+                    int what = 0;
+                    if (a.n2 != b.n2)
+                        what = 16;
+                    else if (a.tsel != b.tsel)
+                        what = 1;
+                    else if (a.kext != b.kext)
+                        what = 2;
+                    else if (a.kprop != b.kprop)
+                        what = 4;
+                    else if (a.kref != b.kref)
+                        what = 8;
+                    if (!what || rows >= cap) continue;
+                    int32_t* o = ids + 5 * size_t(rows);
+                    o[0] = it;
+                    o[1] = color;
+                    o[2] = int32_t(c);
+                    o[3] = what;
+                    o[4] = a.pm;
+                    double* m = margins + 4 * size_t(rows);
+                    m[0] = a.m_draw;
+                    m[1] = a.m_ext;
+                    m[2] = a.m_adopt;
+                    m[3] = a.m_refine;
+                    if (values) {
+                        // FP64 values, FP32 values, FP64 final costs, FP32
+                        // final costs, FP64 now, FP32 now, FP32 kext
+                        double* v = values + 35 * size_t(rows);
+                        for (int i = 0; i < 8; ++i) {
+                            v[i] = a.val[i];
+                            v[8 + i] = b.val[i];
+                            v[16 + i] = a.fc[i];
+                            v[24 + i] = b.fc[i];
+                        }
+                        v[32] = a.now;
+                        v[33] = b.now;
+                        v[34] = b.kext;
+                    }
+                    ++rows;
+                }
+        }
+    if (decisions) *decisions = count;
+    return rows;
 }
 
 // One full PatchMatch pass; precision 64 = oracle, 32 = CPU baseline.

@@ -22,9 +22,9 @@ pytestmark = pytest.mark.gpu
 
 # precision code, bar on the 99.9th percentile of |cost error| over the
 # non-exempt pairs, floor of the depth (0.5%) + normal (1 deg) agreement,
-# floor of the fraction of depths within 0.5%. FP16 is held to the north
-# star's bars (costs to 1e-4, >= 99 % of depths within 0.5 %). BF16 does not
-# meet them (8 significant bits in the tap weights: costs to ~5e-4, ~99.0 %
+# floor of the fraction of depths within 0.5%. FP16 is held to the accuracy
+# target (costs to 1e-4, >= 99 % of depths within 0.5 %). BF16 does not
+# meet it (8 significant bits in the tap weights: costs to ~5e-4, ~99.0 %
 # of depths within 0.5 %); its floors here only guard against regressions.
 MODES = {"fp16": (1, 1e-4, 0.99, 0.99), "bf16": (2, 2e-3, 0.85, 0.98)}
 

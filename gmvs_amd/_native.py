@@ -129,6 +129,8 @@ EXPORTS = (
     "gmvs_cost_batch",
     "gmvs_fuse_workspace_bytes",
     "gmvs_fuse_view",
+    "gmvs_fuse_noalloc_max_pixels",
+    "gmvs_fuse_view_noalloc",
     "gmvs_prior_build",
     "gmvs_prior_device_workspace_bytes",
     "gmvs_prior_build_device",
@@ -213,6 +215,11 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_fuse_view.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
                                    ctypes.POINTER(vp), vp, i32, i32, vp, i32, f32, f32, f32, vp,
                                    vp, vp, sz, vp]
+    lib.gmvs_fuse_noalloc_max_pixels.argtypes = []
+    lib.gmvs_fuse_noalloc_max_pixels.restype = ctypes.c_int64
+    lib.gmvs_fuse_view_noalloc.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
+                                           ctypes.POINTER(vp), vp, i32, i32, vp, i32, f32, f32,
+                                           f32, vp, vp, vp]
     lib.gmvs_prior_device_workspace_bytes.argtypes = [i32, i32]
     lib.gmvs_prior_device_workspace_bytes.restype = sz
     lib.gmvs_prior_build_device.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "abi_common.h"
@@ -193,15 +194,14 @@ __global__ void fuse_scatter(int hw, const int32_t* __restrict__ winner,
     normal[3 * pi + 2] = v.w;
 }
 
-size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct FuseLayout {
     size_t views, mask_copy, winner, rec_idx, rec_val, total;
 };
 
-FuseLayout fuse_layout(int h, int w) {
-    const size_t hw = size_t(h) * w;
-    FuseLayout L;
+constexpr FuseLayout fuse_layout(size_t hw) {
+    FuseLayout L{};
     size_t o = 0;
     L.views = o;
     o = align_up(o + sizeof(FuseView) * (GMVS_MAX_NGB + 1));
@@ -224,7 +224,7 @@ using namespace gmvs;
 
 extern "C" size_t gmvs_fuse_workspace_bytes(int32_t height, int32_t width) {
     if (height <= 0 || width <= 0) return 0;
-    return fuse_layout(height, width).total;
+    return fuse_layout(size_t(height) * width).total;
 }
 
 extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
@@ -255,7 +255,7 @@ extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
                            "fuse_view: src id %d outside the %d mask views", src_ids[i - 1],
                            num_mask_views);
     }
-    const FuseLayout L = fuse_layout(h, w);
+    const FuseLayout L = fuse_layout(size_t(h) * w);
     GMVS_ARG_CHECK(workspace, "fuse_view: workspace is NULL");
     if (workspace_bytes < L.total) {
         set_error("fuse_view: workspace too small: %zu < %zu bytes", workspace_bytes, L.total);
@@ -291,4 +291,75 @@ extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
                        out_normal);
     GMVS_LAUNCH_CHECK();
     return GMVS_OK;
+}
+
+// ---- no-alloc variant ------------------------------------------------------
+// gmvs_fuse_view with its scratch (view table, reference-mask snapshot, winner
+// indices, fused records: 25 B per pixel) in a file-scope device array instead
+// of a caller workspace, for callers that cannot allocate. One copy of the
+// array per device; every call on a device reuses it (each call rewrites all
+// of it that it reads), so calls that overlap on one device race.
+namespace gmvs {
+namespace {
+
+constexpr int64_t kStaticFusePixels = 2048 * 1536;
+constexpr size_t kStaticFuseBytes = fuse_layout(size_t(kStaticFusePixels)).total;
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+}  // namespace gmvs
+
+static __device__ __align__(256) unsigned char scratch[gmvs::kStaticFuseBytes];
+
+namespace gmvs {
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address).
+int static_scratch(void** out) {
+    int dev = 0;
+    GMVS_HIP_CHECK(hipGetDevice(&dev));
+    GMVS_ARG_CHECK(dev >= 0 && dev < kMaxDevices, "fuse_view_noalloc: device %d beyond %d", dev,
+                   kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        GMVS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        GMVS_ARG_CHECK(p != nullptr, "fuse_view_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GMVS_OK;
+}
+
+}  // namespace
+}  // namespace gmvs
+
+extern "C" int64_t gmvs_fuse_noalloc_max_pixels(void) { return kStaticFusePixels; }
+
+extern "C" int gmvs_fuse_view_noalloc(int32_t num_images, const gmvs_camera* cameras,
+                                      const float* const* depths, const float* const* normals,
+                                      uint8_t* masks, int32_t num_mask_views, int32_t ref_id,
+                                      const int32_t* src_ids, int32_t geom_consistent,
+                                      float reproj_px, float rel_depth, float cos_angle,
+                                      float* out_depth, float* out_normal, void* stream) {
+    clear_error();
+    GMVS_ARG_CHECK(cameras, "fuse_view_noalloc: cameras is NULL");
+    const int64_t hw = int64_t(cameras[0].height) * cameras[0].width;
+    GMVS_ARG_CHECK(hw <= kStaticFusePixels,
+                   "fuse_view_noalloc: %dx%d = %lld pixels exceeds the static scratch limit of "
+                   "%lld pixels (use gmvs_fuse_view with a workspace)",
+                   cameras[0].width, cameras[0].height, (long long)hw,
+                   (long long)kStaticFusePixels);
+    GMVS_ARG_CHECK(!profiling_enabled(),
+                   "fuse_view_noalloc: kernel profiling creates timing events; disable it first");
+    void* ws = nullptr;
+    const int rc = static_scratch(&ws);
+    if (rc != GMVS_OK) return rc;
+    return gmvs_fuse_view(num_images, cameras, depths, normals, masks, num_mask_views, ref_id,
+                          src_ids, geom_consistent, reproj_px, rel_depth, cos_angle, out_depth,
+                          out_normal, ws, kStaticFuseBytes, stream);
 }

@@ -230,6 +230,23 @@ int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras, const float* 
                    float rel_depth, float cos_angle, float* out_depth, float* out_normal,
                    void* workspace, size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference: gmvs_fuse_view for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same arguments without the workspace:
+ * the scratch (25 B per pixel) is a file-scope device array of
+ * gmvs_fuse_noalloc_max_pixels() = 2048*1536 pixels, one per device, held
+ * until the process ends. Larger images are rejected with GMVS_ERR_ARG, as is
+ * a call while kernel profiling is on (it creates timing events). Every call
+ * on a device shares that device's array: calls that overlap on one device
+ * (other streams or host threads) race. Nothing is allocated or freed. */
+int64_t gmvs_fuse_noalloc_max_pixels(void);
+int gmvs_fuse_view_noalloc(int32_t num_images, const gmvs_camera* cameras,
+                           const float* const* depths, const float* const* normals,
+                           uint8_t* masks, int32_t num_mask_views, int32_t ref_id,
+                           const int32_t* src_ids, int32_t geom_consistent, float reproj_px,
+                           float rel_depth, float cos_angle, float* out_depth, float* out_normal,
+                           void* stream);
+
 /* Planar prior construction on the host -- PlanarInit::triangulation and
  * cuda_planar_prior_init (gmvs/lib/src/planar_initer.cu:195-420, 686-745):
  * support points from the pass-1 cost, Delaunay triangulation, optional

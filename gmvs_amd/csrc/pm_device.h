@@ -88,6 +88,73 @@ struct ViewDesc {
     const float* depth;            // source depth map [H*W] or nullptr
 };
 
+// The same factors in FP64, from which the per-tile local frames are formed.
+struct ViewF64 {
+    double A[9];
+    double b[3];
+};
+
+// A view's homography in the local frame of one tile of reference pixels:
+// reference positions are taken relative to the tile origin o, source
+// positions relative to an integer anchor a, so every FP32 coordinate of a
+// tap is a small number (the absolute ones reach thousands of pixels, where
+// one FP32 ulp is ~1e-4 px and the NCC's sensitivity to a tap's position
+// turns that into cost errors of that size). With m^T = n^T Kr^-1 / d:
+//   H'' = T(-a) (A - b m^T) T(o) = A'' - b'' m''^T,
+//   A'' = T(-a) A T(o), b'' = T(-a) b, m'' = (m0, m1, m0 ox + m1 oy + m2),
+// A'' and b'' composed in FP64 (the column A (o, 1) - a (A (o, 1))_z is
+// where the cancellation is). The anchor is the image of the point (cx, cy)
+// under the homography of an anchor plane (given by its m in FP64: the
+// fronto-parallel plane at the middle inverse depth for a tile, the queried
+// plane for a single query), floored, its row moved kAnchorRows up and
+// clamped into the image: a tap's cell index relative to the anchor cell is
+// then non-negative whenever the tap is at least one row below the anchor
+// row, which the clamp-free path checks.
+constexpr int kAnchorRows = 64;
+
+struct Frame {
+    float A[9];
+    float b[3];
+    float ax, ay;  // anchor column and row
+    int base;      // ay * wp + ax, the anchor's cell index
+    int unused;
+};
+
+__device__ inline Frame make_frame(const ViewF64& v, int ox, int oy, double cx, double cy,
+                                   const double am[3], int w, int h, int wp) {
+    const double* A = v.A;
+    const double* b = v.b;
+    const double s = fma(am[0], cx, fma(am[1], cy, am[2]));
+    const double u = fma(A[0], cx, fma(A[1], cy, A[2])) - b[0] * s;
+    const double q = fma(A[3], cx, fma(A[4], cy, A[5])) - b[1] * s;
+    const double z = fma(A[6], cx, fma(A[7], cy, A[8])) - b[2] * s;
+    double X = floor(u / z), Y = floor(q / z) - (double)kAnchorRows;
+    if (!(X >= 0.0)) X = 0.0;  // also NaN
+    if (!(Y >= 0.0)) Y = 0.0;
+    X = fmin(X, (double)(w - 1));
+    Y = fmin(Y, (double)(h - 1));
+    const double o0 = ox, o1 = oy;
+    double Ao[3];
+    for (int i = 0; i < 3; ++i) Ao[i] = fma(A[3 * i], o0, fma(A[3 * i + 1], o1, A[3 * i + 2]));
+    Frame f;
+    for (int j = 0; j < 2; ++j) {
+        f.A[j] = (float)(A[j] - X * A[6 + j]);
+        f.A[3 + j] = (float)(A[3 + j] - Y * A[6 + j]);
+        f.A[6 + j] = (float)A[6 + j];
+    }
+    f.A[2] = (float)(Ao[0] - X * Ao[2]);
+    f.A[5] = (float)(Ao[1] - Y * Ao[2]);
+    f.A[8] = (float)Ao[2];
+    f.b[0] = (float)(b[0] - X * b[2]);
+    f.b[1] = (float)(b[1] - Y * b[2]);
+    f.b[2] = (float)b[2];
+    f.ax = (float)X;
+    f.ay = (float)Y;
+    f.base = (int)Y * wp + (int)X;
+    f.unused = 0;
+    return f;
+}
+
 __device__ inline float depth_from_plane(const RefCam& c, float4 pl, float px, float py) {
     return -pl.w * c.K[0] /
            ((px - c.K[2]) * pl.x + (c.K[0] / c.K[4]) * (py - c.K[5]) * pl.y + c.K[0] * pl.z);
@@ -204,13 +271,30 @@ __device__ inline float4 perturbed_normal(const RefCam& c, float px, float py, f
     return o;
 }
 
-// m = n^T Kr^-1 / d for the rank-1 homography update.
-__device__ inline void plane_m(const RefCam& c, float4 pl, float m[3]) {
-    const float id = 1.0f / pl.w;
-    const float a = pl.x / c.K[0], b = pl.y / c.K[4];
-    m[0] = a * id;
-    m[1] = b * id;
-    m[2] = (pl.z - a * c.K[2] - b * c.K[5]) * id;
+// m of the fronto-parallel plane at inverse depth inv_d, m^T = n^T Kr^-1 / d
+// with n = (0, 0, -1): (0, 0, -inv_d).
+__device__ inline void fronto_m(double inv_d, double m[3]) {
+    m[0] = 0.0;
+    m[1] = 0.0;
+    m[2] = -inv_d;
+}
+
+// m'' of the local frame with origin (ox, oy): m^T = n^T Kr^-1 / d with its
+// constant term moved to the origin,
+//   m'' = (m0, m1, m0 (ox - cx) + m1 (oy - cy) + n_z / d) for the plane
+//   (n, d) and Kr = [fx 0 cx; 0 fy cy; 0 0 1],
+// evaluated in FP64 and rounded once per component. b'' m''^T enters every
+// tap position and b'' m''_2 reaches hundreds of pixels; in FP32 the
+// divisions and the cancelling constant term leave m'' a few ulps off,
+// which moves taps by ~1e-4 px and costs by up to ~1e-4 at 1600x1200.
+__device__ inline void plane_m_local(const RefCam& c, float4 pl, float ox, float oy, float m[3]) {
+    const double d = pl.w;
+    const double a = (double)pl.x / ((double)c.K[0] * d);
+    const double b = (double)pl.y / ((double)c.K[4] * d);
+    m[0] = (float)a;
+    m[1] = (float)b;
+    m[2] = (float)fma(a, (double)ox - (double)c.K[2],
+                      fma(b, (double)oy - (double)c.K[5], (double)pl.z / d));
 }
 
 __device__ inline void homography(const float* __restrict__ A, const float* __restrict__ b,
@@ -342,12 +426,13 @@ struct Hoist {
     using B = TapBlock<NT, L>;
     float wt[B::NR * B::NC];
     float wr[B::NR * B::NC];
+    // positions relative to the tile origin of the local frame:
     float tx[B::NR];  // clamped tap columns of the lane's rows (src 400-405)
     float ty[B::NC];  // clamped tap rows of the lane's columns
     float kx[B::NKX];  // the lane's corners of the group's tap rectangle
     float ky[B::NKY];
     float inv_w, mean_r, var_r, c;
-    float px, py;
+    float px, py;  // the pixel
 };
 
 // The hoisted pixel with its weights read from LDS at each use (pm_refine:
@@ -358,7 +443,7 @@ struct HoistRef {
     using B = TapBlock<NT, L>;
     const float* wt;  // LDS, [B::NR * B::NC]
     const float* wr;
-    float tx[B::NR];
+    float tx[B::NR];  // relative to the tile origin, as in Hoist
     float ty[B::NC];
     float kx[B::NKX];
     float ky[B::NKY];
@@ -366,45 +451,46 @@ struct HoistRef {
     float px, py;
 };
 
-// Fills the tap coordinates of a lane (tx, ty, kx, ky).
+// Fills the tap coordinates of a lane (tx, ty, kx, ky, px, py), relative to
+// the tile origin (ox, oy).
 template <int NT, int L, class HS>
 __device__ inline void hoist_taps(HS& hs, int px, int py, int h, int w, int radius,
-                                  int inc, int sub) {
+                                  int inc, int sub, int ox, int oy) {
     using B = TapBlock<NT, L>;
+    hs.px = (float)(px - ox);
+    hs.py = (float)(py - oy);
 #pragma unroll
     for (int r = 0; r < B::NR; ++r)
-        hs.tx[r] = (float)min(max(px - radius + B::row(sub, r) * inc, 0), w - 1);
+        hs.tx[r] = (float)(min(max(px - radius + B::row(sub, r) * inc, 0), w - 1) - ox);
 #pragma unroll
     for (int q = 0; q < B::NC; ++q)
-        hs.ty[q] = (float)min(max(py - radius + B::col(sub, q) * inc, 0), h - 1);
+        hs.ty[q] = (float)(min(max(py - radius + B::col(sub, q) * inc, 0), h - 1) - oy);
     // (the empty asm keeps the compiler from re-deriving the corners as a
     // lane-dependent pick among tx/ty, which it lowers through scratch memory)
 #pragma unroll
     for (int i = 0; i < B::NKX; ++i) {
         int k = B::kx_row(sub, i);
         asm("" : "+v"(k));
-        hs.kx[i] = (float)min(max(px - radius + k * inc, 0), w - 1);
+        hs.kx[i] = (float)(min(max(px - radius + k * inc, 0), w - 1) - ox);
     }
 #pragma unroll
     for (int i = 0; i < B::NKY; ++i) {
         int k = B::ky_col(sub, i);
         asm("" : "+v"(k));
-        hs.ky[i] = (float)min(max(py - radius + k * inc, 0), h - 1);
+        hs.ky[i] = (float)(min(max(py - radius + k * inc, 0), h - 1) - oy);
     }
 }
 
 template <int NT, int L>
 __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref, int px, int py,
                                  int h, int w, int radius, int inc, float sigma_spatial,
-                                 float sigma_color, int sub) {
+                                 float sigma_color, int sub, int ox, int oy) {
     using B = TapBlock<NT, L>;
-    hs.px = (float)px;
-    hs.py = (float)py;
     const float c = ref[py * w + px];
     hs.c = c;
     const float ss = 2.0f * sigma_spatial * sigma_spatial;
     const float sc = 2.0f * sigma_color * sigma_color;
-    hoist_taps<NT, L, Hoist<NT, L>>(hs, px, py, h, w, radius, inc, sub);
+    hoist_taps<NT, L, Hoist<NT, L>>(hs, px, py, h, w, radius, inc, sub, ox, oy);
     float W = 0.f, R1 = 0.f, R2 = 0.f;
 #pragma unroll
     for (int r = 0; r < B::NR; ++r) {
@@ -412,7 +498,7 @@ __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref
 #pragma unroll
         for (int q = 0; q < B::NC; ++q) {
             const float dj = (float)(-radius + B::col(sub, q) * inc);
-            const float rv = ref[(int)hs.ty[q] * w + (int)hs.tx[r]] - c;
+            const float rv = ref[((int)hs.ty[q] + oy) * w + (int)hs.tx[r] + ox] - c;
             const float wgt = fexp(-sqrtf(di * di + dj * dj) / ss - fabsf(rv) / sc);
             hs.wt[r * B::NC + q] = wgt;
             hs.wr[r * B::NC + q] = wgt * rv;
@@ -555,6 +641,39 @@ struct SrcGrid {
     int fast;    // nonzero: clamp-free taps allowed
 };
 
+// The cells of one view as the taps of one tile read them: clamp-free taps
+// index relative to the frame's anchor cell, clamped taps absolutely.
+struct TapSrc {
+    QuadSrc fast;     // records from the anchor cell on
+    QuadSrc clamped;  // every record of the view's grid
+    float ax, ay;     // the anchor (absolute = local + anchor)
+};
+
+template <int PR>
+__device__ inline TapSrc tap_src(const void* view_cells, uint32_t vcells, const Frame& fr) {
+    TapSrc t;
+    const char* c = reinterpret_cast<const char*>(view_cells);
+    t.fast = quad_src<PR>(c + (size_t)fr.base * Cell<PR>::kBytes, vcells - (uint32_t)fr.base, 0);
+    t.clamped = quad_src<PR>(view_cells, vcells, 0);
+    t.ax = fr.ax;
+    t.ay = fr.ay;
+    return t;
+}
+
+// The same over a descriptor of several views' grids (count records), the
+// view's grid starting at record view_base: the bases are per lane.
+template <int PR>
+__device__ inline TapSrc tap_src_at(const void* cells, uint32_t count, int view_base,
+                                    const Frame& fr) {
+    TapSrc t;
+    t.clamped = quad_src<PR>(cells, count, view_base);
+    t.fast = t.clamped;
+    t.fast.base = view_base + fr.base;
+    t.ax = fr.ax;
+    t.ay = fr.ay;
+    return t;
+}
+
 // Floor to int in one instruction (v_cvt_flr_i32_f32).
 __device__ inline int cvt_flr(float x) {
     int r;
@@ -572,10 +691,14 @@ __device__ inline int cvt_flr(float x) {
 // taps need no clamping: the padded grid gives them the reference's clamped
 // values. Otherwise (and for NaN) false.
 template <int NT, int L, class HS>
-__device__ inline bool corners_in_grid(const HS& hs, const float H[9], float zc, const SrcGrid& g) {
+__device__ inline bool corners_in_grid(const HS& hs, const float H[9], float zc, const SrcGrid& g,
+                                       float ax, float ay) {
     using B = TapBlock<NT, L>;
     constexpr float m = 0.0625f;
-    const float xmax = (float)(g.w + kPad) - m, ymax = (float)(g.h + kPad) - m;
+    // local bounds: the padded grid, and at least one row below the anchor
+    // row (a non-negative cell index relative to the anchor cell)
+    const float xmin = m - ax, xmax = (float)(g.w + kPad) - m - ax;
+    const float ymin = 1.0f + m, ymax = (float)(g.h + kPad) - m - ay;
     bool ok = true;
 #pragma unroll
     for (int i = 0; i < B::NKX; ++i) {
@@ -588,7 +711,7 @@ __device__ inline bool corners_in_grid(const HS& hs, const float H[9], float zc,
             const float rz = __builtin_amdgcn_rcpf(z);
             const float x = fmaf(H[1], hs.ky[j], au) * rz;
             const float y = fmaf(H[4], hs.ky[j], av) * rz;
-            ok = ok && (z * zc > 0.0f) && (x >= m) && (x <= xmax) && (y >= m) && (y <= ymax);
+            ok = ok && (z * zc > 0.0f) && (x >= xmin) && (x <= xmax) && (y >= ymin) && (y <= ymax);
         }
     }
     return ok;
@@ -614,10 +737,13 @@ __device__ inline bool corners_in_grid(const HS& hs, const float H[9], float zc,
 // The taps of one lane in one of the two index forms, accumulated into the
 // moments. NG rows of gathers are in flight together.
 template <int NT, int L, int PR, int NG, bool FAST, class HS>
-__device__ inline void ncc_taps(const HS& hs, const float H[9], const QuadSrc& src,
+__device__ inline void ncc_taps(const HS& hs, const float H[9], const TapSrc& src,
                                 const SrcGrid& g, float& S1, float& S2, float& X) {
     using B = TapBlock<NT, L>;
-    const float wm1 = (float)(g.w - 1), hm1 = (float)(g.h - 1), wpf = (float)g.wp;
+    // clamp bounds of the local indices, the reference's [0, n - 1]
+    const float xlo = -src.ax, xhi = (float)(g.w - 1) - src.ax;
+    const float ylo = -src.ay, yhi = (float)(g.h - 1) - src.ay;
+    const float wpf = (float)g.wp;
 #pragma unroll
     for (int r0 = 0; r0 < B::NR; r0 += NG) {
         Cell<PR> q[NG * B::NC];
@@ -635,10 +761,12 @@ __device__ inline void ncc_taps(const HS& hs, const float H[9], const QuadSrc& s
                 const float y = fmaf(H[4], hs.ty[j], av) * rz;
                 const int t = rr * B::NC + j;
                 if constexpr (FAST) {
+                    // y >= 1 and x >= -ax: the index relative to the anchor
+                    // cell is non-negative (the unsigned sum wraps exactly)
                     fx[t] = __builtin_amdgcn_fractf(x);
                     fy[t] = __builtin_amdgcn_fractf(y);
-                    q[t].load(src, (int)(__umul24((unsigned)cvt_flr(y), (unsigned)g.wp) +
-                                         (unsigned)cvt_flr(x)));
+                    q[t].load(src.fast, (int)(__umul24((unsigned)cvt_flr(y), (unsigned)g.wp) +
+                                              (unsigned)cvt_flr(x)));
                 } else {
 #pragma clang fp contract(off)
                     // clamp(trunc(c), 0, n-1) of the reference equals
@@ -646,11 +774,11 @@ __device__ inline void ncc_taps(const HS& hs, const float H[9], const QuadSrc& s
                     // [0, n-1]. The fraction is the rounded x minus the
                     // index (no contraction into x's product), as fract()
                     // forms it in the clamp-free form.
-                    const float xf = __builtin_amdgcn_fmed3f(floorf(x), 0.f, wm1);
-                    const float yf = __builtin_amdgcn_fmed3f(floorf(y), 0.f, hm1);
+                    const float xf = __builtin_amdgcn_fmed3f(floorf(x), xlo, xhi);
+                    const float yf = __builtin_amdgcn_fmed3f(floorf(y), ylo, yhi);
                     fx[t] = x - xf;
                     fy[t] = y - yf;
-                    q[t].load(src, (int)(uint32_t)fmaf(yf, wpf, xf));
+                    q[t].load(src.clamped, (int)(uint32_t)fmaf(yf + src.ay, wpf, xf + src.ax));
                 }
             }
         }
@@ -691,7 +819,7 @@ __device__ inline void ncc_taps(const HS& hs, const float H[9], const QuadSrc& s
 //    or (v00, d01) of the edge cell, whose other terms are multiplied by
 //    exact zeros.
 template <int NT, int L, int PR, bool ALLFIRST, class HS>
-__device__ inline float ncc_eval_t(const HS& hs, const float H[9], const QuadSrc& src,
+__device__ inline float ncc_eval_t(const HS& hs, const float H[9], const TapSrc& src,
                                    const SrcGrid& g) {
     using B = TapBlock<NT, L>;
     constexpr int NG = (ALLFIRST && L == 4) ? B::NR : 1;
@@ -699,10 +827,13 @@ __device__ inline float ncc_eval_t(const HS& hs, const float H[9], const QuadSrc
     const float rcz = __builtin_amdgcn_rcpf(zc);
     const float cx = fmaf(H[0], hs.px, fmaf(H[1], hs.py, H[2])) * rcz;
     const float cy = fmaf(H[3], hs.px, fmaf(H[4], hs.py, H[5])) * rcz;
-    if (cx >= (float)g.w || cx < 0.0f || cy >= (float)g.h || cy < 0.0f) return 2.0f;
+    // the centre inside [0, w) x [0, h) (src 429-432), in local coordinates
+    if (cx >= (float)g.w - src.ax || cx < -src.ax || cy >= (float)g.h - src.ay || cy < -src.ay)
+        return 2.0f;
     // wave-uniform choice (the ballot covers the lanes still active here)
     const bool fast =
-        g.fast && __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L, HS>(hs, H, zc, g)) == 0;
+        g.fast &&
+        __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L, HS>(hs, H, zc, g, src.ax, src.ay)) == 0;
     float S1 = 0.f, S2 = 0.f, X = 0.f;
     if (fast)
         ncc_taps<NT, L, PR, NG, true, HS>(hs, H, src, g, S1, S2, X);
@@ -719,13 +850,13 @@ __device__ inline float ncc_eval_t(const HS& hs, const float H[9], const QuadSrc
 }
 
 template <int NT, int L, int PR, class HS>
-__device__ inline float ncc_eval(const HS& hs, const float H[9], const QuadSrc& src,
+__device__ inline float ncc_eval(const HS& hs, const float H[9], const TapSrc& src,
                                  const SrcGrid& g) {
     return ncc_eval_t<NT, L, PR, false, HS>(hs, H, src, g);
 }
 
 template <int NT, int L, int PR, class HS>
-__device__ inline float ncc_eval_mlp(const HS& hs, const float H[9], const QuadSrc& src,
+__device__ inline float ncc_eval_mlp(const HS& hs, const float H[9], const TapSrc& src,
                                      const SrcGrid& g) {
     return ncc_eval_t<NT, L, PR, true, HS>(hs, H, src, g);
 }

@@ -72,6 +72,7 @@ struct PMConst {
     RefCam rc;
     const float* ref;             // reference intensities [H*W]
     const ViewDesc* views;        // [nv]
+    const ViewF64* views64;       // [nv], the homography factors in FP64
     const void* cells;            // packed cells of all views, [nv][(H+kPad)*(W+kPad)]
     const float* init_depth;      // INIT_DEPTH_NORMAL
     const float* init_normal;     // INIT_DEPTH_NORMAL, world frame
@@ -103,6 +104,21 @@ __device__ inline void count_work(unsigned long long* ctr, uint32_t ncc, uint32_
     if ((threadIdx.x & 63) == 0) {
         if (ncc) atomicAdd(ctr, (unsigned long long)ncc);
         if (geom) atomicAdd(ctr + 1, (unsigned long long)geom);
+    }
+}
+
+// The local frames (see Frame) of one tile of reference pixels for every
+// view of the problem, one thread per view: reference positions relative to
+// the tile origin (ox, oy), source positions relative to the image of the
+// tile's centre under the fronto-parallel plane at the middle inverse depth.
+// The block synchronises before the frames are read.
+__device__ inline void tile_frames(const PMConst& P, int ox, int oy, int cols, int rows,
+                                   Frame* s_fr) {
+    if ((int)threadIdx.x < P.nv) {
+        double am[3];
+        fronto_m(0.5 * (1.0 / (double)P.dmin + 1.0 / (double)P.dmax), am);
+        s_fr[threadIdx.x] = make_frame(P.views64[threadIdx.x], ox, oy, ox + 0.5 * cols,
+                                       oy + 0.5 * rows, am, P.w, P.h, P.wp);
     }
 }
 
@@ -212,7 +228,8 @@ __global__ void pm_setup_tables(WorkTables t, WorkTables* __restrict__ dst) {
 // One single-thread launch per source view; the cameras travel as kernel
 // arguments.
 __global__ void pm_setup_view(gmvs_camera r, gmvs_camera s, const void* cells,
-                              const float* depth, ViewDesc* __restrict__ out) {
+                              const float* depth, ViewDesc* __restrict__ out,
+                              ViewF64* __restrict__ out64) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double Cr[3], Cs[3];
     for (int i = 0; i < 3; ++i) {
@@ -236,11 +253,16 @@ __global__ void pm_setup_view(gmvs_camera r, gmvs_camera s, const void* cells,
         for (int j = 0; j < 3; ++j)
             M[3 * i + j] = Rrel[3 * i] * Ki[j] + Rrel[3 * i + 1] * Ki[3 + j] + Rrel[3 * i + 2] * Ki[6 + j];
     ViewDesc d;
+    ViewF64 d64;
     for (int i = 0; i < 3; ++i) {
-        for (int j = 0; j < 3; ++j)
-            d.A[3 * i + j] = (float)(Ks[3 * i] * M[j] + Ks[3 * i + 1] * M[3 + j] + Ks[3 * i + 2] * M[6 + j]);
-        d.b[i] = (float)(Ks[3 * i] * trel[0] + Ks[3 * i + 1] * trel[1] + Ks[3 * i + 2] * trel[2]);
+        for (int j = 0; j < 3; ++j) {
+            d64.A[3 * i + j] = Ks[3 * i] * M[j] + Ks[3 * i + 1] * M[3 + j] + Ks[3 * i + 2] * M[6 + j];
+            d.A[3 * i + j] = (float)d64.A[3 * i + j];
+        }
+        d64.b[i] = Ks[3 * i] * trel[0] + Ks[3 * i + 1] * trel[1] + Ks[3 * i + 2] * trel[2];
+        d.b[i] = (float)d64.b[i];
     }
+    *out64 = d64;
     for (int i = 0; i < 9; ++i) {
         d.K[i] = s.K[i];
         d.R[i] = s.R[i];
@@ -291,7 +313,8 @@ __global__ void pm_pack_quads(const float* __restrict__ img, void* __restrict__ 
 template <int NT, int L, int PR>
 __device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restrict__ planes,
                                     float* __restrict__ costs, uint32_t* __restrict__ sel,
-                                    float (*s_c)[kTilePix * L], uint32_t& ncc) {
+                                    float (*s_c)[kTilePix * L], const Frame* s_fr, int ox, int oy,
+                                    uint32_t& ncc) {
     const int TX = P.tile.tx(), TY = P.tile.ty();
     const int tx = (P.w + TX - 1) / TX;
     const int slot = threadIdx.x / L, sub = threadIdx.x % L;
@@ -350,15 +373,18 @@ __device__ inline void pm_init_body(const PMConst& P, int tile, float4* __restri
     }
 
     Hoist<NT, L> hs;
-    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub);
+    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub, ox,
+                     oy);
     float m[3];
-    plane_m(rc, pl, m);
+    plane_m_local(rc, pl, (float)ox, (float)oy, m);
     int valid = 0;
     for (int v = 0; v < P.nv; ++v) {
         const ViewDesc& vd = P.views[v];
+        const Frame& fr = s_fr[v];
         float H[9];
-        homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.vcells, 0), P.grid());
+        homography(fr.A, fr.b, m, H);
+        const float cv =
+            ncc_eval<NT, L, PR>(hs, H, tap_src<PR>(vd.cells, P.vcells, fr), P.grid());
         s_c[v][threadIdx.x] = cv;
         valid += (cv < 2.0f);
     }
@@ -404,6 +430,7 @@ __global__ __launch_bounds__(kTilePix * L) void pm_init(const PMProblem* __restr
                                                   uint32_t* __restrict__ queue,
                                                   unsigned long long* work) {
     __shared__ float s_c[kMaxViews][kTilePix * L];
+    __shared__ Frame s_fr[kMaxViews];
     __shared__ int s_item;
     const WorkTable& t = tabs->t[WT_FULL];
     const int total = t.start[t.n];
@@ -414,7 +441,13 @@ __global__ __launch_bounds__(kTilePix * L) void pm_init(const PMProblem* __restr
         if (item < 0) break;
         const int p = problem_of(t, item);
         const PMProblem& pr = probs[p];
-        pm_init_body<NT, L, PR>(pr.P, item - t.start[p], pr.planes, pr.costs, pr.sel, s_c, ncc);
+        const int tile = item - t.start[p];
+        const int TX = pr.P.tile.tx(), TY = pr.P.tile.ty();
+        const int ntx = (pr.P.w + TX - 1) / TX;
+        const int ox = (tile % ntx) * TX, oy = (tile / ntx) * TY;
+        tile_frames(pr.P, ox, oy, TX, TY, s_fr);  // next_item's barrier retired the readers
+        __syncthreads();
+        pm_init_body<NT, L, PR>(pr.P, tile, pr.planes, pr.costs, pr.sel, s_c, s_fr, ox, oy, ncc);
     }
     if (work) count_work(work + 2 * K_INIT, ncc, 0);
 }
@@ -620,7 +653,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
                                           const float* __restrict__ costs, float* __restrict__ ca,
                                           float4* __restrict__ cplane, uint8_t* __restrict__ cflag,
                                           int NA, float (*s_m)[3][kTilePix * L],
-                                          const float (*s_ab)[12], uint32_t& ncc) {
+                                          const Frame* s_fr, int ox, int oy, uint32_t& ncc) {
     const Active a = active_pixel<L>(P, tile, color);
     if (!a.ok) return;
     const bool lead = a.sub == 0;
@@ -633,7 +666,8 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
         ncc += (uint32_t)((__popc(flags) + 1) * P.nv);
     }
     Hoist<NT, L> hs;
-    hoist_ref<NT, L>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub);
+    hoist_ref<NT, L>(hs, P.ref, a.px, a.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, a.sub,
+                     ox, oy);
     // hypotheses 0-7: the flagged candidates; 8: the pixel's own current
     // plane against every view (the selection weights the selected ones)
     pos[8] = a.py * P.w + a.px;
@@ -647,7 +681,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
             const float4 pl = planes[pos[k]];
             if (lead && k < 8) cplane[(size_t)k * NA + a.aidx] = pl;
             float m[3];
-            plane_m(P.rc, pl, m);
+            plane_m_local(P.rc, pl, (float)ox, (float)oy, m);
             s_m[k][0][threadIdx.x] = m[0];
             s_m[k][1][threadIdx.x] = m[1];
             s_m[k][2][threadIdx.x] = m[2];
@@ -657,8 +691,8 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     // wave's gathers in L1
     for (int v = 0; v < P.nv; ++v) {
         const ViewDesc& vd = P.views[v];
-        const QuadSrc src = quad_src<PR>(vd.cells, P.vcells, 0);
-        const float* ab = s_ab[v];  // the view's A (9) and b (3), LDS broadcast reads
+        const Frame& fr = s_fr[v];  // LDS broadcast reads
+        const TapSrc src = tap_src<PR>(vd.cells, P.vcells, fr);
         // a wave-uniform hypothesis index (pixels without hypothesis k sit
         // out that iteration): LDS and cost-table offsets stay scalar
 #pragma unroll 1
@@ -674,7 +708,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
             }
             const float m[3] = {s_m[k][0][threadIdx.x], s_m[k][1][threadIdx.x], s_m[k][2][threadIdx.x]};
             float H[9];
-            homography(ab, ab + 9, m, H);
+            homography(fr.A, fr.b, m, H);
             const float cv = ncc_eval_mlp<NT, L, PR>(hs, H, src, P.grid());
             if (lead) ca[((size_t)k * P.nv + v) * NA + a.aidx] = cv;
         }
@@ -688,27 +722,25 @@ __global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(4)
                                                         unsigned long long* work) {
     __shared__ int s_item;
     __shared__ float s_m[9][3][kTilePix * L];
-    __shared__ float s_ab[kMaxViews][12];
+    __shared__ Frame s_fr[kMaxViews];
     const WorkTable& t = tabs->t[WT_HALF];
     const int total = t.start[t.n];
     uint32_t ncc = 0;
-    int q = blockIdx.x % kXcds, tries = 0, loaded = -1;
+    int q = blockIdx.x % kXcds, tries = 0;
     for (;;) {
         const int item = next_item(queue, total, q, tries, s_item);
         if (item < 0) break;
         const int p = problem_of(t, item);
         const PMProblem& pr = probs[p];
-        if (p != loaded) {  // block-uniform; next_item's barrier retired the readers
-            for (int i = threadIdx.x; i < pr.P.nv * 12; i += kTilePix * L) {
-                const ViewDesc& vd = pr.P.views[i / 12];
-                const int j = i % 12;
-                s_ab[i / 12][j] = j < 9 ? vd.A[j] : vd.b[j - 9];
-            }
-            __syncthreads();
-            loaded = p;
-        }
-        pm_candidates_body<NT, L, PR>(pr.P, item - t.start[p], color, pr.planes, pr.costs, pr.ca,
-                                      pr.cplane, pr.cflag, pr.na, s_m, s_ab, ncc);
+        const int tile = item - t.start[p];
+        // the tile spans 2 TX image columns (TX active ones) and TY rows
+        const int TX = pr.P.tile.tx(), TY = pr.P.tile.ty();
+        const int ntx = (pr.P.wh + TX - 1) / TX;
+        const int ox = 2 * (tile % ntx) * TX, oy = (tile / ntx) * TY;
+        tile_frames(pr.P, ox, oy, 2 * TX, TY, s_fr);  // next_item's barrier retired the readers
+        __syncthreads();
+        pm_candidates_body<NT, L, PR>(pr.P, tile, color, pr.planes, pr.costs, pr.ca, pr.cplane,
+                                      pr.cflag, pr.na, s_m, s_fr, ox, oy, ncc);
     }
     if (work) count_work(work + 2 * K_CANDIDATES, ncc, 0);
 }
@@ -1107,6 +1139,7 @@ struct RefineSmem {
     float hw[kK2Pix][L][2 * NW];  // hoisted w and w * (r - c) per (pixel, lane)
     float hp[kK2Pix][4];               // 1 / sum w, mean_r, var_r, centre intensity
     float4 tp[kRefine][kK2Pix];        // refinement hypotheses
+    float4 tm[kRefine][kK2Pix];        // their m'' in the tile's frame (w unused)
     int nunits;
 };
 
@@ -1127,8 +1160,8 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
                                       float4* __restrict__ planes, float* __restrict__ costs,
                                       const float4* __restrict__ rtp,
                                       const uint32_t* __restrict__ rsw, int NA, RefineSmem<NT, L>& S,
-                                      const ViewDesc* s_views, uint16_t* units, float* res,
-                                      uint32_t& ncc, uint32_t& geom) {
+                                      const ViewDesc* s_views, const Frame* s_fr, int ox, int oy,
+                                      uint16_t* units, float* res, uint32_t& ncc, uint32_t& geom) {
     constexpr int NW = RefineSmem<NT, L>::NW;
     const RefCam& rc = P.rc;
     const int g = threadIdx.x / L, sub = threadIdx.x % L;
@@ -1136,13 +1169,18 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
     for (int i = threadIdx.x; i < kK2Pix * kRefine; i += kTilePix * L) {
         const int sl = i % kK2Pix, f = i / kK2Pix;
         const Active a = active_slot(P, tile, color, sl);
-        S.tp[f][sl] = a.ok ? rtp[(size_t)f * NA + a.aidx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 tp = a.ok ? rtp[(size_t)f * NA + a.aidx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float m[3];
+        plane_m_local(rc, tp, (float)ox, (float)oy, m);
+        S.tp[f][sl] = tp;
+        S.tm[f][sl] = make_float4(m[0], m[1], m[2], 0.f);
     }
     // hoisted reference side of this lane group's pixel
     const Active ag = active_slot(P, tile, color, g);
     if (ag.ok) {
         Hoist<NT, L> hs;
-        hoist_ref<NT, L>(hs, P.ref, ag.px, ag.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub);
+        hoist_ref<NT, L>(hs, P.ref, ag.px, ag.py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub,
+                         ox, oy);
 #pragma unroll
         for (int i = 0; i < NW; ++i) {
             S.hw[g][sub][i] = hs.wt[i];
@@ -1185,24 +1223,23 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
         HoistRef<NT, L> hs;
         hs.wt = &S.hw[sl][sub][0];
         hs.wr = &S.hw[sl][sub][NW];
-        hoist_taps<NT, L, HoistRef<NT, L>>(hs, a.px, a.py, P.h, P.w, P.radius, P.inc, sub);
+        hoist_taps<NT, L, HoistRef<NT, L>>(hs, a.px, a.py, P.h, P.w, P.radius, P.inc, sub, ox, oy);
         hs.inv_w = S.hp[sl][0];
         hs.mean_r = S.hp[sl][1];
         hs.var_r = S.hp[sl][2];
         hs.c = S.hp[sl][3];
-        hs.px = fpx;
-        hs.py = fpy;
         const ViewDesc& vd = s_views[v];
-        const QuadSrc all_views = quad_src<PR>(P.cells, (uint32_t)(P.nv * P.vcells), v * P.vcells);
+        const Frame& fr = s_fr[v];
+        const TapSrc src =
+            tap_src_at<PR>(P.cells, (uint32_t)(P.nv * P.vcells), v * P.vcells, fr);
 #pragma unroll 1
         for (int k = 0; k < kRefine; ++k) {
-            const float4 tp = S.tp[k][sl];
-            float m[3];
-            plane_m(rc, tp, m);
+            const float4 tm = S.tm[k][sl];
+            const float m[3] = {tm.x, tm.y, tm.z};
             float H[9];
-            homography(vd.A, vd.b, m, H);
-            float x = ncc_eval_mlp<NT, L, PR>(hs, H, all_views, P.grid());
-            if (P.geom) x = x + 0.1f * geom_cost(rc, vd, tp, fpx, fpy);
+            homography(fr.A, fr.b, m, H);
+            float x = ncc_eval_mlp<NT, L, PR>(hs, H, src, P.grid());
+            if (P.geom) x = x + 0.1f * geom_cost(rc, vd, S.tp[k][sl], fpx, fpy);
             if (sub == 0) res[(k * kK2Pix + sl) * P.nv + v] = x;
         }
         if (sub == 0) {
@@ -1278,6 +1315,7 @@ __global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(PR
                                                    int nvmax, uint32_t* __restrict__ queue,
                                                    unsigned long long* work) {
     __shared__ RefineSmem<NT, L> S;
+    __shared__ Frame s_fr[kMaxViews];
     __shared__ int s_item;
     extern __shared__ __align__(16) unsigned char ref_dyn[];
     ViewDesc* s_views = reinterpret_cast<ViewDesc*>(ref_dyn);
@@ -1299,8 +1337,13 @@ __global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(PR
             for (int i = threadIdx.x; i < n; i += kTilePix * L) dst[i] = src[i];
             loaded = p;
         }
-        pm_refine_body<NT, L, PR>(pr.P, item - t.start[p], color, pr.planes, pr.costs, pr.rtp,
-                                  pr.rsw, pr.na, S, s_views, units, res, ncc, geom);
+        const int tile = item - t.start[p];
+        const int TX = pr.P.tile.tx(), TY = pr.P.tile.ty();
+        const int ntx = (pr.P.wh + TX - 1) / TX;
+        const int ox = 2 * (tile % ntx) * TX, oy = (tile / ntx) * TY;
+        tile_frames(pr.P, ox, oy, 2 * TX, TY, s_fr);  // the body's first barrier orders it
+        pm_refine_body<NT, L, PR>(pr.P, tile, color, pr.planes, pr.costs, pr.rtp, pr.rsw, pr.na, S,
+                                  s_views, s_fr, ox, oy, units, res, ncc, geom);
     }
     if (work) count_work(work + 2 * K_REFINE, ncc, geom);
 }
@@ -1438,15 +1481,31 @@ __global__ __launch_bounds__(kTilePix * L) void pm_cost_batch(PMConst P, int n,
             for (int v = 0; v < P.nv; ++v) out[(size_t)q * P.nv + v] = __builtin_nanf("");
         return;
     }
+    // the local frame of the query: reference positions relative to the
+    // pixel, source positions anchored at the pixel's image under the
+    // queried plane itself
     Hoist<NT, L> hs;
-    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub);
+    hoist_ref<NT, L>(hs, P.ref, px, py, P.h, P.w, P.radius, P.inc, P.sigma_s, P.sigma_c, sub, px,
+                     py);
+    const float4 pl = planes[q];
     float m[3];
-    plane_m(P.rc, planes[q], m);
+    plane_m_local(P.rc, pl, (float)px, (float)py, m);
+    double am[3];
+    {
+        const RefCam& c = P.rc;
+        const double id = 1.0 / (double)pl.w;
+        const double a = (double)pl.x / c.K[0], b = (double)pl.y / c.K[4];
+        am[0] = a * id;
+        am[1] = b * id;
+        am[2] = ((double)pl.z - a * c.K[2] - b * c.K[5]) * id;
+    }
     for (int v = 0; v < P.nv; ++v) {
         const ViewDesc& vd = P.views[v];
+        const Frame fr = make_frame(P.views64[v], px, py, px, py, am, P.w, P.h, P.wp);
         float H[9];
-        homography(vd.A, vd.b, m, H);
-        const float cv = ncc_eval<NT, L, PR>(hs, H, quad_src<PR>(vd.cells, P.vcells, 0), P.grid());
+        homography(fr.A, fr.b, m, H);
+        const float cv =
+            ncc_eval<NT, L, PR>(hs, H, tap_src_at<PR>(vd.cells, P.vcells, 0, fr), P.grid());
         if (sub == 0) out[(size_t)q * P.nv + v] = cv;
     }
 }
@@ -1457,7 +1516,7 @@ namespace {
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct PMLayout {
-    size_t views, quads, sel, ca, cplane, cflag, rtp, rsw, total;
+    size_t views, views64, quads, sel, ca, cplane, cflag, rtp, rsw, total;
 };
 
 bool valid_precision(int precision) {
@@ -1477,6 +1536,8 @@ PMLayout pm_layout(int h, int w, int num_images, bool sweep, int precision) {
     size_t o = 0;
     L.views = o;
     o = align_up(o + sizeof(ViewDesc) * nv);
+    L.views64 = o;
+    o = align_up(o + sizeof(ViewF64) * nv);
     L.quads = o;
     o = align_up(o + cell_bytes(precision) * grid_cells(h, w) * nv);
     L.sel = o;
@@ -1535,10 +1596,12 @@ int setup_views(int num_images, const gmvs_camera* cams, const float* const* ima
     char* cells = ws + L.quads;
     const size_t view_bytes = grid_cells(h, w) * Cell<PR>::kBytes;
     ViewDesc* views = reinterpret_cast<ViewDesc*>(ws + L.views);
+    ViewF64* views64 = reinterpret_cast<ViewF64*>(ws + L.views64);
     const int blocks = int((grid_cells(h, w) + 255) / 256);
     for (int v = 0; v < nv; ++v) {
         GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_view, dim3(1), dim3(64), 0, st, cams[0], cams[v + 1],
-                    (const void*)(cells + view_bytes * v), depths ? depths[v + 1] : nullptr, views + v);
+                    (const void*)(cells + view_bytes * v), depths ? depths[v + 1] : nullptr, views + v,
+                    views64 + v);
         GMVS_LAUNCH_CHECK();
         GMVS_LAUNCH(K_PACK, st, pm_pack_quads<PR>, dim3(blocks), dim3(256), 0, st, images[v + 1],
                     (void*)(cells + view_bytes * v), h, w);
@@ -1551,7 +1614,7 @@ int setup_views(int num_images, const gmvs_camera* cams, const float* const* ima
 std::atomic<int> g_fast_taps{1};
 
 PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, const ViewDesc* views,
-                   const gmvs_params& prm, int radius) {
+                   const ViewF64* views64, const gmvs_params& prm, int radius) {
     PMConst P;
     std::memset(&P, 0, sizeof(P));
     const gmvs_camera& r = cams[0];
@@ -1567,6 +1630,7 @@ PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, co
     P.rc.w = r.width;
     P.ref = ref;
     P.views = views;
+    P.views64 = views64;
     P.nv = num_images - 1;
     P.h = r.height;
     P.w = r.width;
@@ -1743,7 +1807,8 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
         std::memset(&pr, 0, sizeof(pr));
         PMConst& P = pr.P;
         P = make_const(A.num_images, A.cameras, A.images[0],
-                       reinterpret_cast<const ViewDesc*>(rw + L.views), A.params, radius[i]);
+                       reinterpret_cast<const ViewDesc*>(rw + L.views),
+                       reinterpret_cast<const ViewF64*>(rw + L.views64), A.params, radius[i]);
         P.cells = rw + L.quads;
         P.init_depth = A.depths ? A.depths[0] : nullptr;
         P.init_normal = A.ref_normal;
@@ -1876,7 +1941,8 @@ int cost_batch(int num_images, const gmvs_camera* cameras, const float* const* i
                float* costs, char* ws, const PMLayout& L, hipStream_t st) {
     if (int rc = setup_views<PR>(num_images, cameras, images, nullptr, ws, L, st)) return rc;
     PMConst P = make_const(num_images, cameras, images[0],
-                           reinterpret_cast<const ViewDesc*>(ws + L.views), params, radius);
+                           reinterpret_cast<const ViewDesc*>(ws + L.views),
+                           reinterpret_cast<const ViewF64*>(ws + L.views64), params, radius);
     P.cells = ws + L.quads;
     // the candidate kernel's lane-group width (the costs of a pass's hypotheses)
     const int lg = lanes_of(K_CANDIDATES, PR);

@@ -490,6 +490,10 @@ __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref
     hs.c = c;
     const float ss = 2.0f * sigma_spatial * sigma_spatial;
     const float sc = 2.0f * sigma_color * sigma_color;
+    // the weight's two divisions as products with the reciprocals (weights
+    // within ~2 ulp of the quotient form; the NCC is invariant to their
+    // common scale and insensitive to ulp-level relative changes)
+    const float iss = 1.0f / ss, isc = 1.0f / sc;
     hoist_taps<NT, L, Hoist<NT, L>>(hs, px, py, h, w, radius, inc, sub, ox, oy);
     float W = 0.f, R1 = 0.f, R2 = 0.f;
 #pragma unroll
@@ -499,7 +503,7 @@ __device__ inline void hoist_ref(Hoist<NT, L>& hs, const float* __restrict__ ref
         for (int q = 0; q < B::NC; ++q) {
             const float dj = (float)(-radius + B::col(sub, q) * inc);
             const float rv = ref[((int)hs.ty[q] + oy) * w + (int)hs.tx[r] + ox] - c;
-            const float wgt = fexp(-sqrtf(di * di + dj * dj) / ss - fabsf(rv) / sc);
+            const float wgt = fexp(-__builtin_amdgcn_sqrtf(di * di + dj * dj) * iss - fabsf(rv) * isc);
             hs.wt[r * B::NC + q] = wgt;
             hs.wr[r * B::NC + q] = wgt * rv;
             W += wgt;
@@ -689,17 +693,20 @@ __device__ inline int cvt_flr(float x) {
 // far above the FP32 rounding of a tap position) every tap's floor cell is
 // inside the padded grid and none is left of or above the image, so the
 // taps need no clamping: the padded grid gives them the reference's clamped
-// values. Otherwise (and for NaN) false.
+// values. Returns the mask of the wave's lanes whose corner passes (a NaN
+// fails every comparison); inactive lanes read as failing.
 template <int NT, int L, class HS>
-__device__ inline bool corners_in_grid(const HS& hs, const float H[9], float zc, const SrcGrid& g,
-                                       float ax, float ay) {
+__device__ inline uint64_t corners_in_grid(const HS& hs, const float H[9], float zc,
+                                           const SrcGrid& g, float ax, float ay) {
     using B = TapBlock<NT, L>;
     constexpr float m = 0.0625f;
     // local bounds: the padded grid, and at least one row below the anchor
     // row (a non-negative cell index relative to the anchor cell)
     const float xmin = m - ax, xmax = (float)(g.w + kPad) - m - ax;
     const float ymin = 1.0f + m, ymax = (float)(g.h + kPad) - m - ay;
-    bool ok = true;
+    // each comparison's lane mask straight from its compare (no per-lane
+    // boolean arithmetic)
+    uint64_t ok = ~0ull;
 #pragma unroll
     for (int i = 0; i < B::NKX; ++i) {
         const float au = fmaf(H[0], hs.kx[i], H[2]);
@@ -711,7 +718,9 @@ __device__ inline bool corners_in_grid(const HS& hs, const float H[9], float zc,
             const float rz = __builtin_amdgcn_rcpf(z);
             const float x = fmaf(H[1], hs.ky[j], au) * rz;
             const float y = fmaf(H[4], hs.ky[j], av) * rz;
-            ok = ok && (z * zc > 0.0f) && (x >= xmin) && (x <= xmax) && (y >= ymin) && (y <= ymax);
+            ok &= __builtin_amdgcn_ballot_w64(z * zc > 0.0f) & __builtin_amdgcn_ballot_w64(x >= xmin) &
+                  __builtin_amdgcn_ballot_w64(x <= xmax) & __builtin_amdgcn_ballot_w64(y >= ymin) &
+                  __builtin_amdgcn_ballot_w64(y <= ymax);
         }
     }
     return ok;
@@ -832,8 +841,8 @@ __device__ inline float ncc_eval_t(const HS& hs, const float H[9], const TapSrc&
         return 2.0f;
     // wave-uniform choice (the ballot covers the lanes still active here)
     const bool fast =
-        g.fast &&
-        __builtin_amdgcn_ballot_w64(!corners_in_grid<NT, L, HS>(hs, H, zc, g, src.ax, src.ay)) == 0;
+        g.fast && (__builtin_amdgcn_read_exec() &
+                   ~corners_in_grid<NT, L, HS>(hs, H, zc, g, src.ax, src.ay)) == 0;
     float S1 = 0.f, S2 = 0.f, X = 0.f;
     if (fast)
         ncc_taps<NT, L, PR, NG, true, HS>(hs, H, src, g, S1, S2, X);

@@ -463,8 +463,8 @@ __device__ inline Active active_pixel(const PMConst& P, int tile, int color) {
     const int TX = P.tile.tx(), TY = P.tile.ty();
     const int tx = (P.wh + TX - 1) / TX;
     const int slot = threadIdx.x / L;
-    const int ax = (tile % tx) * TX + (slot % TX);
-    const int y = (tile / tx) * TY + (slot / TX);
+    const int ax = (tile % tx) * TX + (slot & (TX - 1));
+    const int y = (tile / tx) * TY + (slot >> P.tile.shift);
     Active a;
     a.sub = threadIdx.x % L;
     a.py = y;
@@ -1122,8 +1122,8 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
 __device__ inline Active active_slot(const PMConst& P, int tile, int color, int slot) {
     const int TX = P.tile.tx(), TY = P.tile.ty();
     const int tx = (P.wh + TX - 1) / TX;
-    const int ax = (tile % tx) * TX + (slot % TX);
-    const int y = (tile / tx) * TY + (slot / TX);
+    const int ax = (tile % tx) * TX + (slot & (TX - 1));
+    const int y = (tile / tx) * TY + (slot >> P.tile.shift);
     Active a;
     a.sub = 0;
     a.py = y;

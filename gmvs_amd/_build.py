@@ -46,6 +46,7 @@ SOURCES = {
     "fusion.hip": ["-ffp-contract=off"],
     "post.hip": ["-ffp-contract=off"],
     "prior_gpu.hip": ["-ffp-contract=off"],
+    "measure.hip": [],
 }
 
 

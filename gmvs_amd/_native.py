@@ -131,6 +131,8 @@ EXPORTS = (
     "gmvs_fuse_view",
     "gmvs_fuse_noalloc_max_pixels",
     "gmvs_fuse_view_noalloc",
+    "gmvs_gather_peak_buffer_bytes",
+    "gmvs_gather_peak",
     "gmvs_prior_build",
     "gmvs_prior_device_workspace_bytes",
     "gmvs_prior_build_device",
@@ -220,6 +222,9 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_fuse_view_noalloc.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
                                            ctypes.POINTER(vp), vp, i32, i32, vp, i32, f32, f32,
                                            f32, vp, vp, vp]
+    lib.gmvs_gather_peak_buffer_bytes.argtypes = [i32, i32]
+    lib.gmvs_gather_peak_buffer_bytes.restype = sz
+    lib.gmvs_gather_peak.argtypes = [i32, i32, i32, vp, sz, ctypes.POINTER(ctypes.c_double), vp]
     lib.gmvs_prior_device_workspace_bytes.argtypes = [i32, i32]
     lib.gmvs_prior_device_workspace_bytes.restype = sz
     lib.gmvs_prior_build_device.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,
@@ -303,3 +308,21 @@ def set_fast_taps(enable: bool) -> None:
 def set_lane_width(kernel: int, lanes: int) -> None:
     """Lanes per pixel (2 or 4; 0 = default) of a PatchMatch NCC kernel (TUNE_*)."""
     check("gmvs_set_lane_width", lib.gmvs_set_lane_width(kernel, lanes))
+
+
+def gather_peak(cell_bytes: int, window_cells: int = 0, rounds: int = 2000) -> float:
+    """Peak gather rate (GB/s) of the NCC tap loads, measured on the current
+    device (gmvs_gather_peak); the window defaults to 2 KB per block."""
+    import torch
+
+    win = window_cells or 2048 // cell_bytes
+    nbytes = lib.gmvs_gather_peak_buffer_bytes(cell_bytes, win)
+    if nbytes == 0:
+        raise ValueError("gather_peak: bad cell size or window")
+    buf = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+    out = ctypes.c_double(0.0)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check("gmvs_gather_peak", lib.gmvs_gather_peak(cell_bytes, win, rounds,
+                                                     ctypes.c_void_p(buf.data_ptr()), nbytes,
+                                                     ctypes.byref(out), stream))
+    return out.value

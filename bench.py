@@ -31,8 +31,11 @@ if HERE not in sys.path:
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0      # MI355X published HBM3E bandwidth
-FLOPS_PER_NCC = 1150.0     # algorithmic FP32 flops of one 36-tap bilateral NCC
-FLOPS_PER_GEOM = 110.0     # algorithmic FP32 flops of one forward-backward reprojection
+# Algorithmic FP32 operations of one 36-tap bilateral NCC: per tap ~22 (map,
+# divide, index and fraction, bilinear, three weighted moments) plus ~40 for
+# the per-NCC homography, centre test and epilogue: 36 * 22 + 40.
+OPS_PER_NCC = 36 * 22 + 40
+OPS_PER_GEOM = 110.0       # one forward-backward reprojection
 GATHER_BYTES_PER_NCC = 36 * 4 * 4  # 36 taps x 4 bilinear texels x 4 B
 
 
@@ -64,28 +67,136 @@ def candidate_ncc_count(h: int, w: int, nv: int) -> int:
     return int(flags.sum()) * nv // 2
 
 
-def cpu_baseline(sc, threads: int):
-    """FP32 CPU port of the reference loop (OpenMP) on a bounded sample:
-    view 0 of the workload, plain pass, 2 iterations."""
+def host_cpu_info() -> dict:
+    """nproc, the OpenMP thread count used and the lscpu model line."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                             "CPU(s)", "CPU max MHz"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def host_threads() -> int:
+    """All host threads this process may use: OMP_NUM_THREADS when the
+    environment sets it (the machine's share for this GPU), else the
+    affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(sc, labels, threads: int, iterations: int = 8, geom_iterations: int = 2):
+    """The FP32 OpenMP port of the reference loop on one reference view of
+    the workload, each pass of the benchmark's mix timed on its own at one
+    iteration: plain pass, host prior construction (superpixel filter),
+    prior pass, and a geometric pass against the scene's depth maps with
+    1 % noise. Median of 3 runs after a warm-up run. The rate is the
+    benchmark step's pixel-hypotheses of one view over the time the port
+    takes for them, each pass's time per pixel-hypothesis applied to that
+    pass's share of the step (iterations / geom_iterations per pass) plus
+    the host prior. Returns (Mpix-hyp/s, per-pass seconds, runs)."""
     import numpy as np
 
+    from gmvs_amd import src as S
+    from gmvs_amd.lib.build import planar_initer as PI
     from oracle import oracle as O
 
-    ids = [0] + sc.pairs[0][:9]
+    ids = [0] + list(sc.pairs[0][:9])
     cams = [O.make_camera(sc.cameras[i].K, sc.cameras[i].R, sc.cameras[i].t, sc.cameras[i].height,
                           sc.cameras[i].width, sc.cameras[i].depth_min, sc.cameras[i].depth_max)
             for i in ids]
     imgs = [sc.images[i].cpu().numpy() for i in ids]
-    dmin = float(np.float32(float(np.float32(sc.cameras[0].depth_min)) * 0.6))
-    dmax = float(np.float32(float(np.float32(sc.cameras[0].depth_max)) * 1.2))
-    iters = 2
-    p = O.make_params(max_iterations=iters, depth_min=dmin, depth_max=dmax)
+    c0 = sc.cameras[0]
+    dmin = float(np.float32(float(np.float32(c0.depth_min)) * 0.6))
+    dmax = float(np.float32(float(np.float32(c0.depth_max)) * 1.2))
     h, w = imgs[0].shape
-    t0 = time.perf_counter()
-    O.pm_run(cams, imgs, p, precision=32, threads=threads, rng_view=0)
-    dt = time.perf_counter() - t0
-    ph = h * w * (1 + 14 * iters)
-    return ph / dt / 1e6, dt
+    rng = np.random.default_rng(3407)
+    depths = [(sc.depth[i].cpu().numpy() * (1 + 0.01 * rng.standard_normal((h, w)))).astype(np.float32)
+              for i in ids]
+    normal = sc.normal[0].cpu().numpy()
+    ccam = S.load_camera(c0.K.reshape(-1), c0.R.reshape(-1), c0.t, h, w, c0.depth_min,
+                         c0.depth_max)._c
+    lab = labels[0].cpu().numpy() if hasattr(labels[0], "cpu") else np.asarray(labels[0])
+
+    def one_view():
+        t = {}
+        t0 = time.perf_counter()
+        p1 = O.make_params(max_iterations=1, depth_min=dmin, depth_max=dmax)
+        o1 = O.pm_run(cams, imgs, p1, precision=32, threads=threads, rng_view=0)
+        t["plain"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pp, pm, _ = PI.build_prior(ccam, o1["depth"], o1["cost"], lab, dmin, dmax,
+                                   superpixel_filter=True)
+        t["prior_build"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p2 = O.make_params(max_iterations=1, depth_min=dmin, depth_max=dmax, planar_prior=True,
+                           superpixel_filter=True)
+        O.pm_run(cams, imgs, p2, precision=32, threads=threads, init_mode=2, prior_planes=pp,
+                 plane_masks=pm.astype(np.uint32), planes=o1["planes"], costs=o1["cost"],
+                 rng_view=0)
+        t["prior"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p3 = O.make_params(max_iterations=1, depth_min=dmin, depth_max=dmax,
+                           geom_consistency=True)
+        O.pm_run(cams, imgs, p3, precision=32, threads=threads, init_mode=1, geom=True,
+                 depths=depths, ref_normal=normal, rng_view=0)
+        t["geom"] = time.perf_counter() - t0
+        return t
+
+    one_view()  # warm-up
+    runs = [one_view() for _ in range(3)]
+    med = {k: float(np.median([r[k] for r in runs])) for k in runs[0]}
+    ph1 = h * w * (1 + 14 * 1)  # pixel-hypotheses of a one-iteration pass
+    share = {"plain": h * w * (1 + 14 * iterations), "prior": h * w * (1 + 14 * iterations),
+             "geom": h * w * (1 + 14 * geom_iterations)}
+    t_step = med["prior_build"] + sum(med[k] / ph1 * share[k] for k in share)
+    return sum(share.values()) / t_step / 1e6, med, runs
+
+
+def cpu_fusion_baseline(fd_in, fn_in, fcams, fpairs, threads: int):
+    """The fusion CPU twin (the sequential restatement of batch_fusion_kernel)
+    on the first views of the fusion workload, in view order: one thread on
+    2 views, then `threads` OpenMP threads on 8 views. Mpts/s of input points."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    d = fd_in.cpu().numpy()
+    n = fn_in.cpu().numpy()
+    ocams = [O.make_camera(c.K, c.R, c.t, c.height, c.width, c.depth_min, c.depth_max)
+             for c in fcams]
+    nv, h, w = d.shape
+    cos_t = float(np.float32(math.cos(math.radians(30.0))))
+    res = {}
+    for label, nthreads, nviews in (("sequential", 1, 2), ("openmp", threads, 8)):
+        O.set_threads(nthreads)
+        masks = np.zeros((nv, h, w), np.uint8)
+        t0 = time.perf_counter()
+        for ref in range(nviews):
+            ids = [ref] + list(fpairs[ref])
+            O.fuse_view([ocams[i] for i in ids], [d[i] for i in ids], [n[i] for i in ids], masks,
+                        ref, ids[1:], 3, 1.0, 0.01, cos_t)
+        dt = time.perf_counter() - t0
+        res[label] = {"mpts_per_s": round(nviews * h * w / dt / 1e6, 2), "threads": nthreads,
+                      "views": nviews, "s": round(dt, 2)}
+    O.set_threads(threads)
+    return res
 
 
 def fusion_workload(device: str):
@@ -441,31 +552,47 @@ def main() -> int:
     avg_ms = dom["ms"] / dom["launches"]
     ncc_per_launch = dom["ncc"] / dom["launches"]
     geom_per_launch = dom["geom"] / dom["launches"]
-    flops = ncc_per_launch * FLOPS_PER_NCC + geom_per_launch * FLOPS_PER_GEOM
-    ach = flops / (avg_ms * 1e-3) / 1e12
-    # HBM bytes per launch of that kernel from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE passes over this workload (scaled by a copy of
-    # known size); counters cannot be read inside a timed run
-    traffic, hbm = None, None
+    ops = ncc_per_launch * OPS_PER_NCC + geom_per_launch * OPS_PER_GEOM
+    ach = ops / (avg_ms * 1e-3) / 1e12
+    # the tap gathers of that kernel against the measured peak of the same
+    # loads from the vector L1 cache (FP32 cells: 16 B per tap)
+    gather_ach = ncc_per_launch * GATHER_BYTES_PER_NCC / (avg_ms * 1e-3) / 1e9
+    gather_peak = N.gather_peak(16)
+    log(f"gather peak (16-byte cells, L1-resident): {gather_peak:.0f} GB/s")
+    # HBM bytes and VALU issue utilisation per launch of that kernel from the
+    # committed rocprofv3 passes over this workload (FETCH_SIZE / WRITE_SIZE
+    # scaled by a copy of known size; SQ_INSTS_VALU per launch over one wave
+    # instruction per CU per clock at 2.4 GHz); counters cannot be read inside
+    # a timed run, so each figure names the commit its profile was taken at
+    traffic, hbm, valu = None, None, None
     tpath = os.path.join(HERE, "profiles", "pm_traffic_acmp_10v_1600x1200.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            tk = json.load(f)["kernels"].get(dom_name)
+            tj = json.load(f)
+        tk = tj["kernels"].get(dom_name)
         if tk:
             traffic = tk["traffic_bytes_per_launch"]
             gbs = traffic / (avg_ms * 1e-3) / 1e9
             hbm = {"achieved_GBps": round(gbs, 1), "peak_GBps": PEAK_HBM_GBS,
-                   "frac": round(gbs / PEAK_HBM_GBS, 4)}
+                   "frac": round(gbs / PEAK_HBM_GBS, 4), "profile": os.path.basename(tpath),
+                   "profile_commit": tj.get("commit")}
+            if "valu_insts_per_launch" in tk:
+                util = tk["valu_insts_per_launch"] / (256 * 2.4e9 * tk["avg_ms"] * 1e-3)
+                valu = {"issue_util": round(util, 4), "insts_per_launch": tk["valu_insts_per_launch"],
+                        "insts_per_ncc": round(tk["valu_insts_per_launch"] / ncc_per_launch, 2),
+                        "profile": os.path.basename(tpath), "profile_commit": tj.get("commit")}
     roof = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-            "hbm": hbm,
+            "ops_per_ncc": OPS_PER_NCC, "hbm": hbm, "valu": valu,
+            "gather": {"achieved_GBps": round(gather_ach, 1), "peak_GBps": round(gather_peak, 1),
+                       "frac": round(gather_ach / gather_peak, 4),
+                       "bytes_per_ncc": GATHER_BYTES_PER_NCC},
             "kernel": dom_name, "avg_ms": round(avg_ms, 4),
-            "ncc_per_launch": int(ncc_per_launch), "geom_per_launch": int(geom_per_launch),
-            "gather_GBps": round(ncc_per_launch * GATHER_BYTES_PER_NCC / (avg_ms * 1e-3) / 1e9, 1)}
+            "ncc_per_launch": int(ncc_per_launch), "geom_per_launch": int(geom_per_launch)}
     if "pm_candidates" in stats:
         s = stats["pm_candidates"]
-        log(f"pm_candidates NCCs per launch: counted {s['ncc'] / s['launches']:.0f}, "
-            f"analytic {candidate_ncc_count(h, w, 9)}")
+        log(f"pm_candidates NCCs per launch: counted {s['ncc'] / s['launches']:.0f}; per view "
+            f"and half-sweep (analytic) {candidate_ncc_count(h, w, 9)}")
     total_ncc = sum(s["ncc"] for s in stats.values())
     total_kernel_ms = sum(s["ms"] for s in stats.values())
     log(f"NCC evaluations per step {total_ncc // args.steps}, "
@@ -526,14 +653,25 @@ def main() -> int:
         fusion["post"] = post_fusion(fsc, od, on, not args.no_cpu_baseline)
         log(f"post-fusion gather and text: {fusion['post']}")
         del od, on, mk
+        if not args.no_cpu_baseline:
+            log("fusion CPU twin ...")
+            fusion["cpu_twin"] = cpu_fusion_baseline(fd_in, fn_in, fcams, fpairs, host_threads())
+            log(f"fusion CPU twin: {fusion['cpu_twin']}")
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         log(f"CPU baseline (FP32 OpenMP port, {threads} threads) ...")
-        rate, dt = cpu_baseline(sc, threads)
+        rate, med, runs = cpu_baseline(sc, labels, threads)
         cpu = {"value": round(rate, 3), "unit": "Mpix-hyp/s", "cores": threads, "kind": "port",
-               "sample": f"view 0 of the workload, plain pass, 2 iterations ({dt:.1f} s)"}
+               "sample": ("view 0 of the workload: plain pass, host prior, prior pass and "
+                          "geometric pass at 1 iteration each, median of 3 runs after a warm-up "
+                          f"({sum(med.values()):.1f} s per run); the rate weights each pass's "
+                          "time per pixel-hypothesis by its share of the step (8, 8 and 2 "
+                          "iterations) and adds the host prior"),
+               "pass_s": {k: round(v, 3) for k, v in med.items()},
+               "runs_s": [round(sum(r.values()), 2) for r in runs], "host": host_cpu_info(),
+               "gpu_vs_cpu": round(value / rate, 1)}
         log(f"CPU baseline: {cpu}")
 
     line = {

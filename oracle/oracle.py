@@ -81,10 +81,13 @@ def _load():
                                       vp, vp, vp, vp, vp]
     lib.oracle_geom_batch.argtypes = [ctypes.POINTER(OCamera), ctypes.c_int, ctypes.POINTER(vp),
                                       ctypes.c_int, vp, vp, vp]
+    lib.oracle_geom_batch32.argtypes = lib.oracle_geom_batch.argtypes
     lib.oracle_pm_run.argtypes = [ctypes.c_int, ctypes.POINTER(OPMArgs)]
     lib.oracle_pm_lockstep.argtypes = [ctypes.POINTER(OPMArgs), ctypes.c_int32, vp, vp, vp, vp]
     lib.oracle_pm_lockstep.restype = ctypes.c_int32
     lib.oracle_max_threads.restype = ctypes.c_int
+    lib.oracle_set_threads.argtypes = [ctypes.c_int]
+    lib.oracle_set_threads.restype = None
     lib.oracle_fuse_view.argtypes = [ctypes.POINTER(OCamera), ctypes.c_int, ctypes.POINTER(vp),
                                      ctypes.POINTER(vp), vp, ctypes.c_int, vp, ctypes.c_int,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
@@ -189,13 +192,13 @@ def cost_batch(cams: Sequence[OCamera], images: Sequence[np.ndarray], params: OP
 
 
 def geom_batch(cams: Sequence[OCamera], depths: Sequence[np.ndarray], xy: np.ndarray,
-               planes: np.ndarray) -> np.ndarray:
+               planes: np.ndarray, precision: int = 64) -> np.ndarray:
     ds = [_f32(a) for a in depths]
     xy = np.ascontiguousarray(xy, np.int32)
     planes = _f32(planes)
     out = np.zeros((xy.shape[0], len(ds) - 1), np.float64)
-    rc = lib().oracle_geom_batch(_cams(cams), len(ds), _ptrs(ds), xy.shape[0], _ptr(xy),
-                                 _ptr(planes), _ptr(out))
+    fn = lib().oracle_geom_batch if precision == 64 else lib().oracle_geom_batch32
+    rc = fn(_cams(cams), len(ds), _ptrs(ds), xy.shape[0], _ptr(xy), _ptr(planes), _ptr(out))
     assert rc == 0, rc
     return out
 
@@ -441,3 +444,8 @@ def t_prior_score(cost, depth, depth_prior, n_prior, n, dmin, dmax) -> float:
     a, b = _f64(n_prior), _f64(n)
     return float(lib().oracle_t_prior_score(float(cost), float(depth), float(depth_prior), _ptr(a),
                                             _ptr(b), float(dmin), float(dmax)))
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the fusion twin (and of any call without a thread count)."""
+    lib().oracle_set_threads(int(n))

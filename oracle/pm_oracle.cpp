@@ -1077,6 +1077,23 @@ int check_args(const OPMArgs* a) {
     return 0;
 }
 
+// Geometric-consistency cost for n (pixel, plane) queries, FP64 (precision
+// 64, the oracle) or FP32 (precision 32, the CPU port's arithmetic).
+template <class T>
+void geom_batch_t(const OCamera* cams, int num_images, const float* const* depths, int n,
+                  const int32_t* xy, const float* planes, double* out) {
+    const int nv = num_images - 1;
+    std::vector<Cam<T>> c;
+    for (int i = 0; i < num_images; ++i) c.push_back(to_cam<T>(cams[i]));
+    for (int q = 0; q < n; ++q) {
+        const V4<T> pl{T(planes[4 * q]), T(planes[4 * q + 1]), T(planes[4 * q + 2]),
+                       T(planes[4 * q + 3])};
+        for (int v = 0; v < nv; ++v)
+            out[size_t(q) * nv + v] =
+                double(geom_cost<T>(depths[v + 1], c[0], c[v + 1], pl, xy[2 * q], xy[2 * q + 1]));
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1109,19 +1126,17 @@ int oracle_cost_batch(const OCamera* cams, int num_images, const float* const* i
     return 0;
 }
 
-// FP64 geometric-consistency cost for n (pixel, plane) queries.
 int oracle_geom_batch(const OCamera* cams, int num_images, const float* const* depths, int n,
                       const int32_t* xy, const float* planes, double* out) {
     if (!cams || !depths || num_images < 2) return 1;
-    const int nv = num_images - 1;
-    std::vector<Cam<double>> c;
-    for (int i = 0; i < num_images; ++i) c.push_back(to_cam<double>(cams[i]));
-    for (int q = 0; q < n; ++q) {
-        const V4<double> pl{planes[4 * q], planes[4 * q + 1], planes[4 * q + 2], planes[4 * q + 3]};
-        for (int v = 0; v < nv; ++v)
-            out[size_t(q) * nv + v] =
-                geom_cost<double>(depths[v + 1], c[0], c[v + 1], pl, xy[2 * q], xy[2 * q + 1]);
-    }
+    geom_batch_t<double>(cams, num_images, depths, n, xy, planes, out);
+    return 0;
+}
+
+int oracle_geom_batch32(const OCamera* cams, int num_images, const float* const* depths, int n,
+                        const int32_t* xy, const float* planes, double* out) {
+    if (!cams || !depths || num_images < 2) return 1;
+    geom_batch_t<float>(cams, num_images, depths, n, xy, planes, out);
     return 0;
 }
 
@@ -1320,6 +1335,15 @@ int oracle_pm_run(int precision, const OPMArgs* args) {
     omp_set_num_threads(saved);
 #endif
     return 0;
+}
+
+// OpenMP threads of the calls that take no thread count (the fusion twin).
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
 }
 
 int oracle_max_threads() {

@@ -179,16 +179,15 @@ def fp16_cells_exact(images) -> bool:
     integers FP16 represents exactly, and GMVS_PREC_FP16 -- whose bilinear
     sample widens the FP16 operands into FP32 FMAs -- returns the FP32
     path's results bit for bit with half the gather bytes."""
+    oks = []
     for im in images:
         t = im if isinstance(im, torch.Tensor) else torch.as_tensor(np.asarray(im))
         t = t.float()
         if t.numel() == 0:
             continue
-        if not bool(torch.isfinite(t).all()) or float(t.min()) < 0.0 or float(t.max()) > 1024.0:
-            return False
-        if not bool((t == torch.round(t)).all()):
-            return False
-    return True
+        # NaN fails every comparison; one host read for all images
+        oks.append(((t >= 0.0) & (t <= 1024.0) & (t == torch.round(t))).all())
+    return bool(torch.stack(oks).all()) if oks else True
 
 
 def resolve_precision(precision, images) -> int:
@@ -307,6 +306,7 @@ class PatchMatcher:
         self._cameras: List[Camera] = []
         self.images_host = self.depths_host = self.normals_host = self.costs_host = None
         self._dev = None
+        self._prec = None  # (self.precision, resolved GMVS_PREC_*) for the current images
         self._ws = _Workspace()
 
     def add_samples(self, problems: Sequence[Problem], cameras: Sequence[Camera],
@@ -321,6 +321,14 @@ class PatchMatcher:
         self.images_host, self.depths_host = images, depths
         self.normals_host, self.costs_host = normals, costs
         self._dev = None
+        self._prec = None
+
+    def _resolved_precision(self, images) -> int:
+        """resolve_precision once per set of images and precision setting
+        ("auto" inspects every image of the scene)."""
+        if self._prec is None or self._prec[0] != self.precision:
+            self._prec = (self.precision, resolve_precision(self.precision, images))
+        return self._prec[1]
 
     def _upload(self):
         if self._dev is None:
@@ -355,7 +363,7 @@ class PatchMatcher:
         run_pm(cams, [images[i] for i in ids], params, init_mode=N.INIT_DEPTH_NORMAL, geom=True,
                rng_view=ref, planes=planes, costs=costs, depths=[depths[i] for i in ids],
                ref_normal=normals[ref], out_depth=out_d, out_normal=out_n, out_cost=out_c,
-               workspace=self._ws, precision=resolve_precision(self.precision, images))
+               workspace=self._ws, precision=self._resolved_precision(images))
         if verbose:
             for it in range(self.params.max_iterations):
                 print(f"iteration: {it}")

@@ -95,6 +95,36 @@ def test_fp16_cells_exact_and_auto_precision():
     assert S.resolve_precision(N.PREC_AUTO, frac) == N.PREC_FP32
     assert not S.fp16_cells_exact([u8[0] * 8])      # beyond 1024
     assert not S.fp16_cells_exact([u8[0] - 1.0])    # negative
+    nan = u8[1].clone()
+    nan[3, 4] = float("nan")
+    assert not S.fp16_cells_exact([u8[0], nan])
+    assert S.fp16_cells_exact([torch.zeros(0), u8[2]])  # empty images are skipped
     assert S.resolve_precision("bf16", u8) == N.PREC_BF16
     with pytest.raises(ValueError):
         S.resolve_precision("fp8", u8)
+
+
+def test_patch_matcher_resolves_precision_once_per_images(monkeypatch):
+    import numpy as np
+    import torch
+
+    from gmvs_amd import _native as N
+    from gmvs_amd import src as S
+
+    calls = []
+    real = S.resolve_precision
+    monkeypatch.setattr(S, "resolve_precision", lambda p, im: calls.append(p) or real(p, im))
+    u8 = torch.from_numpy(np.random.default_rng(1).integers(0, 256, (2, 12, 16)).astype(np.float32))
+    pm = S.PatchMatcher()
+    pm.precision = "auto"
+    assert pm._resolved_precision(u8) == N.PREC_FP16
+    assert pm._resolved_precision(u8) == N.PREC_FP16
+    assert len(calls) == 1
+    pm.precision = "fp32"                      # a new setting is resolved again
+    assert pm._resolved_precision(u8) == N.PREC_FP32
+    assert len(calls) == 2
+    pm.precision = "auto"
+    z = torch.zeros(2, 12, 16)
+    pm.add_samples([], [], u8 + 0.5, z, torch.zeros(2, 12, 16, 3), z)  # new images
+    assert pm._resolved_precision(pm.images_host) == N.PREC_FP32
+    assert len(calls) == 3

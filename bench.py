@@ -471,7 +471,12 @@ def main() -> int:
     ap.add_argument("--no-fusion", action="store_true")
     ap.add_argument("--variants", default="plain,fp16,bf16",
                     help="other runs of the same workload: plain (FP32, one view per call), "
-                         "fp16 / bf16 (batched, 16-bit samples); comma list, empty for none")
+                         "fp16 / bf16 (batched, 16-bit samples), fused (batched FP32 with the "
+                         "view selection fused into the candidate kernel); comma list, empty "
+                         "for none")
+    ap.add_argument("--fuse-select", action="store_true",
+                    help="run the view selection as the candidate kernel's tile epilogue in the "
+                         "measured pipeline (gmvs_set_select_fusion; identical results)")
     ap.add_argument("--workload", default="acmp10", choices=("acmp10", "scenes4"),
                     help="acmp10: the 10-view ACMP pipeline (default); scenes4: 4 scenes x 12 "
                          "views of uneven sizes in one batched HIP graph")
@@ -499,6 +504,7 @@ def main() -> int:
     from gmvs_amd.pipeline import MultiViewPipeline, cameras_from_synthetic, fuse_maps, pixel_hypotheses
 
     dev = "cuda"
+    N.set_select_fusion(args.fuse_select)
     if args.copy_probe_mib > 0:
         nel = args.copy_probe_mib * (1 << 18)
         src = torch.ones(nel, device=dev)
@@ -604,13 +610,17 @@ def main() -> int:
     specs = []
     for v in [v for v in args.variants.split(",") if v]:
         if v == "plain":
-            specs.append(("fp32_per_view", N.PREC_FP32, False))
+            specs.append(("fp32_per_view", N.PREC_FP32, False, False))
+        elif v == "fused":
+            # selection and adoption as the candidate kernel's tile epilogue
+            specs.append(("fp32_batched_fused_select", N.PREC_FP32, True, True))
         else:
-            specs.append((f"{v}_batched", N.PRECISIONS[v], True))
+            specs.append((f"{v}_batched", N.PRECISIONS[v], True, False))
     variants = {}
-    for name, prec, batched in specs:
+    for name, prec, batched, fuse_sel in specs:
         vp = MultiViewPipeline(sc.images, cams, pairs, labels=labels, prior_threads=8,
                                precision=prec, batched=batched)
+        N.set_select_fusion(fuse_sel)
         for i in range(args.warmup):
             step(vp)
         torch.cuda.synchronize()
@@ -619,6 +629,7 @@ def main() -> int:
             vfd = step(vp)
         torch.cuda.synchronize()
         vt = time.perf_counter() - t0
+        N.set_select_fusion(args.fuse_select)
         vval = ph_step * args.steps / vt / 1e6
         vfused = int((vfd > 0).sum().item())
         # the fused depth map against the FP32 batched run's, bit for bit
@@ -688,6 +699,7 @@ def main() -> int:
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "acmp_10v_1600x1200_prior_geom_fusion", "batched": True,
+                   "select_fused": bool(args.fuse_select),
                    "plain_fp32_per_view": plain, "views": n,
                    "height": h, "width": w, "sources": 9, "iterations": 8, "geom_iterations": 2,
                    "superpixels": 2000, "planes": 24, "pixel_hypotheses_per_step": ph_step,

@@ -125,6 +125,7 @@ EXPORTS = (
     "gmvs_set_blocks_per_cu",
     "gmvs_set_lane_width",
     "gmvs_set_fast_taps",
+    "gmvs_set_select_fusion",
     "gmvs_cost_workspace_bytes",
     "gmvs_cost_batch",
     "gmvs_fuse_workspace_bytes",
@@ -208,6 +209,7 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_set_blocks_per_cu.argtypes = [i32, i32]
     lib.gmvs_set_lane_width.argtypes = [i32, i32]
     lib.gmvs_set_fast_taps.argtypes = [i32]
+    lib.gmvs_set_select_fusion.argtypes = [i32]
     lib.gmvs_cost_workspace_bytes.argtypes = [i32, i32, i32, i32]
     lib.gmvs_cost_workspace_bytes.restype = sz
     lib.gmvs_cost_batch.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
@@ -303,6 +305,12 @@ def camera_array(cams: Sequence[CCamera]):
 def set_fast_taps(enable: bool) -> None:
     """Clamp-free NCC tap indexing on (default) or off; results are identical."""
     check("gmvs_set_fast_taps", lib.gmvs_set_fast_taps(1 if enable else 0))
+
+
+def set_select_fusion(enable: bool) -> None:
+    """View selection and adoption as the candidate kernel's tile epilogue
+    (True) or as their own launch (False, the default); identical results."""
+    check("gmvs_set_select_fusion", lib.gmvs_set_select_fusion(1 if enable else 0))
 
 
 def set_lane_width(kernel: int, lanes: int) -> None:

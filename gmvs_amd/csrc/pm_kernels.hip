@@ -138,7 +138,8 @@ struct PMProblem {
     float* ca;              // [9][nv][na] candidate NCC table
     float4* cplane;         // [8][na] candidate planes
     uint8_t* cflag;         // [na] candidate flags
-    float4* rtp;            // [5][na] refinement hypotheses
+    float4* rtp;            // [6][na] refinement hypotheses, then the adopted
+                            // plane (fused selection only)
     uint32_t* rsw;          // [SW_NUM][na] selection record
     float* out_depth;       // [H*W] or nullptr
     float* out_normal;      // [H*W*3] or nullptr
@@ -458,6 +459,22 @@ struct Active {
     bool ok;
 };
 
+// Pixel `slot` of a tile of one colour's active pixels (slot = lane group in
+// the NCC layout; pm_refine and the fused selection epilogue).
+__device__ inline Active active_slot(const PMConst& P, int tile, int color, int slot) {
+    const int TX = P.tile.tx(), TY = P.tile.ty();
+    const int tx = (P.wh + TX - 1) / TX;
+    const int ax = (tile % tx) * TX + (slot & (TX - 1));
+    const int y = (tile / tx) * TY + (slot >> P.tile.shift);
+    Active a;
+    a.sub = 0;
+    a.py = y;
+    a.px = 2 * ax + ((y + color) & 1);
+    a.aidx = y * P.wh + ax;
+    a.ok = (y < P.h) && (ax < P.wh) && (a.px < P.w);
+    return a;
+}
+
 template <int L>
 __device__ inline Active active_pixel(const PMConst& P, int tile, int color) {
     const int TX = P.tile.tx(), TY = P.tile.ty();
@@ -715,35 +732,6 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
     }
 }
 
-template <int NT, int L, int PR>
-__global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(4))) void pm_candidates(const PMProblem* __restrict__ probs,
-                                                        const WorkTables* __restrict__ tabs,
-                                                        int color, uint32_t* __restrict__ queue,
-                                                        unsigned long long* work) {
-    __shared__ int s_item;
-    __shared__ float s_m[9][3][kTilePix * L];
-    __shared__ Frame s_fr[kMaxViews];
-    const WorkTable& t = tabs->t[WT_HALF];
-    const int total = t.start[t.n];
-    uint32_t ncc = 0;
-    int q = blockIdx.x % kXcds, tries = 0;
-    for (;;) {
-        const int item = next_item(queue, total, q, tries, s_item);
-        if (item < 0) break;
-        const int p = problem_of(t, item);
-        const PMProblem& pr = probs[p];
-        const int tile = item - t.start[p];
-        // the tile spans 2 TX image columns (TX active ones) and TY rows
-        const int TX = pr.P.tile.tx(), TY = pr.P.tile.ty();
-        const int ntx = (pr.P.wh + TX - 1) / TX;
-        const int ox = 2 * (tile % ntx) * TX, oy = (tile / ntx) * TY;
-        tile_frames(pr.P, ox, oy, 2 * TX, TY, s_fr);  // next_item's barrier retired the readers
-        __syncthreads();
-        pm_candidates_body<NT, L, PR>(pr.P, tile, color, pr.planes, pr.costs, pr.ca, pr.cplane,
-                                      pr.cflag, pr.na, s_m, s_fr, ox, oy, ncc);
-    }
-    if (work) count_work(work + 2 * K_CANDIDATES, ncc, 0);
-}
 
 // ------------------------------------------------------- half-sweep, B ---
 // The second half of a half-sweep runs as two kernels:
@@ -825,6 +813,7 @@ enum SelWord {
     SW_DRAND,        // refinement depths (float bits)
     SW_DPERT,
     SW_DCUR,
+    SW_COST,         // the adopted cost (float bits), fused selection only
     SW_NUM
 };
 
@@ -845,6 +834,327 @@ __device__ inline Active active_of_index(const PMConst& P, int color, int aidx) 
     return a;
 }
 
+// Joint view selection (src 915-981), candidate adoption with plain or
+// planar-prior scoring and the optional geometric term (src 983-1040,
+// lib 1185-1334), and the 5 refinement hypotheses (src 595-676,
+// lib 684-811) of active pixel aidx (valid, of colour `color`), from the
+// candidate table of pm_candidates. cdf: this pixel's column of a
+// float[nv][cdf_stride] scratch table; views: the problem's view
+// descriptors (read for the geometric term). defer: leave planes/costs
+// unchanged and record the adopted plane and cost in the workspace
+// (rtp[kRefine], SW_COST) for pm_refine to commit -- required when the
+// selection runs inside pm_candidates, whose other tiles still read the
+// colour's planes and costs (the candidate search reaches pixels of the
+// pixel's own colour, and the half-sweep reads a snapshot). Returns the
+// geometric terms evaluated.
+__device__ inline uint32_t select_pixel(const PMProblem& pr, int color, int iter, const Active& a,
+                                        float* cdf, int cdf_stride, const ViewDesc* views,
+                                        bool defer) {
+    const PMConst& P = pr.P;
+    float4* __restrict__ planes = pr.planes;
+    float* __restrict__ costs = pr.costs;
+    uint32_t* __restrict__ sel = pr.sel;
+    const float* __restrict__ ca = pr.ca;
+    const float4* __restrict__ cplane = pr.cplane;
+    const uint8_t* __restrict__ cflag = pr.cflag;
+    float4* __restrict__ rtp = pr.rtp;
+    uint32_t* __restrict__ rsw = pr.rsw;
+    const int NA = pr.na;
+    const int aidx = a.aidx;
+    const int px = a.px, py = a.py;
+    const int c = py * P.w + px;
+    const float fpx = (float)px, fpy = (float)py;
+    const RefCam& rc = P.rc;
+    const int nv = P.nv;
+    const uint32_t flags = cflag[aidx];
+    const Rng rng{(uint32_t)c, (uint32_t)iter, P.seed, P.view};
+
+    // ---- joint view selection (src 915-981)
+    uint32_t nbsel[4];
+    const int nb[4] = {c - P.w, c + P.w, c - 1, c + 1};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) nbsel[i] = ((flags >> (2 * i)) & 1u) ? sel[nb[i]] : 0u;
+    const float tau = 0.8f * fexp((float)(iter * iter) / -90.0f);
+    float psum = 0.f;
+    for (int v = 0; v < nv; ++v) {
+        float pri = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if ((flags >> (2 * i)) & 1u) pri += ((nbsel[i] >> v) & 1u) ? 0.9f : 0.1f;
+        float count = 0.f, tmpw = 0.f;
+        int cf = 0;
+        float xs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xs[k] = ca_entry(ca, k, v, nv, NA, aidx);
+        // branch-free (adding +0 leaves the sums' bits unchanged), so the
+        // loads above are all issued before the first use
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float x = xs[k];
+            const bool in = x < tau;
+            const float e = fexp(x * x / -0.18f);
+            tmpw += in ? e : 0.0f;
+            count += in ? 1.0f : 0.0f;
+            cf += (x > 1.2f);
+        }
+        float p = 0.f;
+        if (count > 2.0f && cf < 3)
+            p = tmpw / count;
+        else if (cf < 3)
+            p = fexp(tau * tau / -0.32f);
+        p = p * pri;
+        cdf[v * cdf_stride] = p;
+        psum += p;
+    }
+    const float inv = 1.0f / psum;
+    float cum = 0.f;
+    for (int v = 0; v < nv; ++v) {
+        cum += cdf[v * cdf_stride] * inv;
+        cdf[v * cdf_stride] = cum;
+    }
+    float rp[15];
+    float u_depth_rand;
+    {
+        uint32_t o[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            rng.block(b, o);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = 4 * b + j;
+                if (s < 15)
+                    rp[s] = u01(o[j]) - FLT_EPSILON;
+                else
+                    u_depth_rand = u01(o[j]);
+            }
+        }
+    }
+    uint32_t done = 0, selmask = 0;
+    uint32_t wnib[4] = {0u, 0u, 0u, 0u};
+    float wsum = 0.f;
+    for (int v = 0; v < nv; ++v) {
+        const float cd = cdf[v * cdf_stride];
+        uint32_t hit = 0;
+#pragma unroll
+        for (int d = 0; d < 15; ++d) hit |= (uint32_t)(cd > rp[d]) << d;
+        const uint32_t fresh = hit & ~done;
+        done |= fresh;
+        const uint32_t cnt = __popc(fresh);
+        if (cnt) {
+            selmask |= 1u << v;
+            const uint32_t add = cnt << (4u * (uint32_t)(v & 7));
+            if ((v >> 3) == 0) wnib[0] |= add;
+            else if ((v >> 3) == 1) wnib[1] |= add;
+            else if ((v >> 3) == 2) wnib[2] |= add;
+            else wnib[3] |= add;
+            wsum += (float)cnt;
+        }
+    }
+
+    // ---- candidate final costs (src 983-1009) and the current
+    // hypothesis re-costed with the new view weights (its NCCs come from
+    // pm_candidates, row 8 of the record). Plain and prior passes walk
+    // the selected views once and read the view's 9 costs together; the
+    // geometric pass adds a reprojection term per (candidate, view).
+    // Every final cost is its weighted sum times one reciprocal of the
+    // weight sum: a candidate that is the current plane must compare
+    // equal to it (the adoption test is strict), and two divisions by
+    // wsum may be lowered to different approximate sequences.
+    float fc[8];
+    const float rws = 1.0f / wsum;
+    float4 cur = planes[c];
+    float cnow;
+    if (!P.geom) {
+        float sk[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sk[k] = 0.f;
+        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+            const int v = __ffs(mm) - 1;
+            const float wv = nib_weight(wnib, v);
+            float xv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) xv[k] = ca_entry(ca, k, v, nv, NA, aidx);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) sk[k] = fmaf(wv, xv[k], sk[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) fc[k] = sk[k] * rws;
+        cnow = sk[8] * rws;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool fk = (flags >> k) & 1u;
+            float4 cp = make_float4(0.f, 0.f, 0.f, 1.f);
+            if (fk) cp = cplane[(size_t)k * NA + aidx];
+            float sk = 0.f;
+            for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+                const int v = __ffs(mm) - 1;
+                const float wv = nib_weight(wnib, v);
+                const float x = ca_entry(ca, k, v, nv, NA, aidx);
+                const float g = fk ? x + 0.1f * geom_cost(rc, views[v], cp, fpx, fpy) : x;
+                sk = fmaf(wv, g, sk);
+            }
+            fc[k] = sk * rws;
+        }
+        float sk = 0.f;
+        for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
+            const int v = __ffs(mm) - 1;
+            const float x = ca_entry(ca, 8, v, nv, NA, aidx);
+            const float wv = nib_weight(wnib, v);
+            sk = fmaf(wv, x + 0.1f * geom_cost(rc, views[v], cur, fpx, fpy), sk);
+        }
+        cnow = sk * rws;
+    }
+    const int kmin = argmin_last8(fc);
+
+    float cost = cnow;
+    uint32_t selc = sel[c];
+    float depth_now = depth_from_plane(rc, cur, fpx, fpy);
+    const float4 cur0 = cur;
+
+    const bool masked = P.prior && P.plane_masks[c] > 0u;
+    float4 pp = make_float4(0.f, 0.f, 0.f, 0.f);
+    float dprior = 0.f;
+    if (P.prior) {
+        pp = P.prior_planes[c];
+        dprior = depth_from_plane(rc, pp, fpx, fpy);
+    }
+    float restricted = 0.f;
+    if (P.prior && masked) {
+        float rs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            rs[k] = 0.f;
+            if ((flags >> k) & 1u) {
+                const float4 cp = cplane[(size_t)k * NA + aidx];
+                rs[k] = prior_score(P, fc[k], depth_from_plane(rc, cp, fpx, fpy), dprior, pp, cp);
+            }
+        }
+        const int kmax = argmax_last8(rs);
+        const float rnow = prior_score(P, cnow, depth_from_plane(rc, cur0, fpx, fpy), dprior, pp, cur0);
+        if ((flags >> kmax) & 1u) {
+            const float4 cp = cplane[(size_t)kmax * NA + aidx];
+            const float db = depth_from_plane(rc, cp, fpx, fpy);
+            if (db >= P.dmin && db <= P.dmax && rs[kmax] > rnow) {
+                depth_now = db;
+                cur = cp;
+                cost = fc[kmax];
+                restricted = rs[kmax];
+                selc = selmask;
+            }
+        }
+    } else if ((flags >> kmin) & 1u) {
+        const float4 cp = cplane[(size_t)kmin * NA + aidx];
+        const float db = depth_from_plane(rc, cp, fpx, fpy);
+        if (db >= P.dmin && db <= P.dmax && fc[kmin] < cnow) {
+            depth_now = db;
+            cur = cp;
+            cost = fc[kmin];
+            // the prior path leaves it (lib 1306-1318) unless
+            // GMVS_COMPAT_FIX_PRIOR_SELECTION
+            if (!P.prior || (P.compat & GMVS_COMPAT_FIX_PRIOR_SELECTION)) selc = selmask;
+        }
+    }
+
+    // ---- refinement hypotheses (src 595-676, lib 684-811)
+    const float pert = 0.02f;
+    float drand;
+    float4 nrand;
+    uint32_t o4[4], o5[4];
+    rng.block(4, o4);  // slots 16..19
+    if (P.prior && masked) {
+        const float dsig = (P.dmax - P.dmin) / 64.0f;
+        drand = u_depth_rand * 6.0f * dsig + (dprior - 3.0f * dsig);
+        rng.block(5, o5);  // slots 20..23
+        nrand = perturbed_normal(rc, fpx, fpy, pp, u01(o5[0]), u01(o5[1]), u01(o5[2]),
+                                 kPiF * (5.0f / 180.0f));
+    } else {
+        drand = u_depth_rand * (P.dmax - P.dmin) + P.dmin;
+        nrand = random_normal(rc, fpx, fpy, rng, RNG_NORMAL, depth_now);
+    }
+    const float dlo = (1.0f - pert) * depth_now;
+    const float dhi = (1.0f + pert) * depth_now;
+    const float dpert = u01(o4[0]) * (dhi - dlo) + dlo;
+    const float4 npert = perturbed_normal(rc, fpx, fpy, cur, u01(o4[1]), u01(o4[2]), u01(o4[3]),
+                                          pert * kPiF);
+    const float dcur = depth_now;
+#pragma unroll
+    for (int i = 0; i < kRefine; ++i) {
+        const float di = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dcur);
+        float4 tp = (i == 0 || i == 4) ? cur : (i == 3 ? npert : nrand);
+        tp.w = dist_to_origin(rc, fpx, fpy, di, tp);
+        rtp[(size_t)i * NA + aidx] = tp;
+    }
+    if (defer) {
+        rtp[(size_t)kRefine * NA + aidx] = cur;
+        rsw[(size_t)SW_COST * NA + aidx] = __float_as_uint(cost);
+    } else {
+        planes[c] = cur;
+        costs[c] = cost;
+    }
+    sel[c] = selc;
+    rsw[(size_t)SW_SELMASK * NA + aidx] = selmask;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rsw[(size_t)(SW_WNIB0 + i) * NA + aidx] = wnib[i];
+    rsw[(size_t)SW_RESTRICTED * NA + aidx] = __float_as_uint(restricted);
+    rsw[(size_t)SW_DRAND * NA + aidx] = __float_as_uint(drand);
+    rsw[(size_t)SW_DPERT * NA + aidx] = __float_as_uint(dpert);
+    rsw[(size_t)SW_DCUR * NA + aidx] = __float_as_uint(dcur);
+    return P.geom ? ((uint32_t)__popc(flags) + 1u) * (uint32_t)__popc(selmask) : 0u;
+}
+
+template <int NT, int L, int PR, bool FUSE>
+__global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(4))) void pm_candidates(const PMProblem* __restrict__ probs,
+                                                        const WorkTables* __restrict__ tabs,
+                                                        int color, int iter,
+                                                        uint32_t* __restrict__ queue,
+                                                        unsigned long long* work) {
+    __shared__ int s_item;
+    __shared__ float s_m[9][3][kTilePix * L];
+    __shared__ Frame s_fr[kMaxViews];
+    const WorkTable& t = tabs->t[WT_HALF];
+    const int total = t.start[t.n];
+    static_assert(sizeof(s_m) >= sizeof(float) * kMaxViews * kTilePix,
+                  "the CDF scratch of the fused selection lives in s_m");
+    uint32_t ncc = 0, geom_count = 0;
+    int q = blockIdx.x % kXcds, tries = 0;
+    for (;;) {
+        const int item = next_item(queue, total, q, tries, s_item);
+        if (item < 0) break;
+        const int p = problem_of(t, item);
+        const PMProblem& pr = probs[p];
+        const int tile = item - t.start[p];
+        // the tile spans 2 TX image columns (TX active ones) and TY rows
+        const int TX = pr.P.tile.tx(), TY = pr.P.tile.ty();
+        const int ntx = (pr.P.wh + TX - 1) / TX;
+        const int ox = 2 * (tile % ntx) * TX, oy = (tile / ntx) * TY;
+        tile_frames(pr.P, ox, oy, 2 * TX, TY, s_fr);  // next_item's barrier retired the readers
+        __syncthreads();
+        pm_candidates_body<NT, L, PR>(pr.P, tile, color, pr.planes, pr.costs, pr.ca, pr.cplane,
+                                      pr.cflag, pr.na, s_m, s_fr, ox, oy, ncc);
+        if constexpr (FUSE) {
+            // the selection as the tile's epilogue: the block's candidate
+            // table and planes are written (the barrier orders the global
+            // writes within the workgroup); wave 0 selects for the tile's
+            // 64 pixels, the CDF scratch in the m'' staging area. The
+            // adopted state goes to the workspace (pm_refine commits it):
+            // other tiles of this launch still read the colour's planes
+            // and costs.
+            __syncthreads();
+            if (threadIdx.x < kTilePix) {
+                const Active a = active_slot(pr.P, tile, color, threadIdx.x);
+                if (a.ok)
+                    geom_count += select_pixel(pr, color, iter, a, &s_m[0][0][0] + threadIdx.x,
+                                               kTilePix, pr.P.views, true);
+            }
+        }
+    }
+    if (work) {
+        count_work(work + 2 * K_CANDIDATES, ncc, 0);
+        if constexpr (FUSE) count_work(work + 2 * K_SELECT, 0u, geom_count);
+    }
+}
+
 // One block per WT_ACTIVE item (256 active pixels of one problem).
 // Dynamic LDS: ViewDesc[nvmax], then the CDF table float[nvmax][kBlock].
 __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict__ probs,
@@ -858,15 +1168,6 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
     const int pidx = problem_of(t, item);
     const PMProblem& pr = probs[pidx];
     const PMConst& P = pr.P;
-    float4* __restrict__ planes = pr.planes;
-    float* __restrict__ costs = pr.costs;
-    uint32_t* __restrict__ sel = pr.sel;
-    const float* __restrict__ ca = pr.ca;
-    const float4* __restrict__ cplane = pr.cplane;
-    const uint8_t* __restrict__ cflag = pr.cflag;
-    float4* __restrict__ rtp = pr.rtp;
-    uint32_t* __restrict__ rsw = pr.rsw;
-    const int NA = pr.na;
     if (P.geom) {
         const int n = P.nv * (int)(sizeof(ViewDesc) / 4);
         const uint32_t* src = reinterpret_cast<const uint32_t*>(P.views);
@@ -877,261 +1178,11 @@ __global__ __launch_bounds__(kBlock) void pm_select(const PMProblem* __restrict_
     uint32_t geom_count = 0;
     const int aidx = (item - t.start[pidx]) * kBlock + threadIdx.x;
     const Active a = active_of_index(P, color, aidx);
-    if (aidx < NA && a.ok) {
-        const int px = a.px, py = a.py;
-        const int c = py * P.w + px;
-        const float fpx = (float)px, fpy = (float)py;
-        const RefCam& rc = P.rc;
-        const int nv = P.nv;
-        const uint32_t flags = cflag[aidx];
-        const Rng rng{(uint32_t)c, (uint32_t)iter, P.seed, P.view};
-        float* cdf = s_cdf + threadIdx.x;
-
-        // ---- joint view selection (src 915-981)
-        uint32_t nbsel[4];
-        const int nb[4] = {c - P.w, c + P.w, c - 1, c + 1};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) nbsel[i] = ((flags >> (2 * i)) & 1u) ? sel[nb[i]] : 0u;
-        const float tau = 0.8f * fexp((float)(iter * iter) / -90.0f);
-        float psum = 0.f;
-        for (int v = 0; v < nv; ++v) {
-            float pri = 0.f;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if ((flags >> (2 * i)) & 1u) pri += ((nbsel[i] >> v) & 1u) ? 0.9f : 0.1f;
-            float count = 0.f, tmpw = 0.f;
-            int cf = 0;
-            float xs[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) xs[k] = ca_entry(ca, k, v, nv, NA, aidx);
-            // branch-free (adding +0 leaves the sums' bits unchanged), so the
-            // loads above are all issued before the first use
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float x = xs[k];
-                const bool in = x < tau;
-                const float e = fexp(x * x / -0.18f);
-                tmpw += in ? e : 0.0f;
-                count += in ? 1.0f : 0.0f;
-                cf += (x > 1.2f);
-            }
-            float p = 0.f;
-            if (count > 2.0f && cf < 3)
-                p = tmpw / count;
-            else if (cf < 3)
-                p = fexp(tau * tau / -0.32f);
-            p = p * pri;
-            cdf[v * kBlock] = p;
-            psum += p;
-        }
-        const float inv = 1.0f / psum;
-        float cum = 0.f;
-        for (int v = 0; v < nv; ++v) {
-            cum += cdf[v * kBlock] * inv;
-            cdf[v * kBlock] = cum;
-        }
-        float rp[15];
-        float u_depth_rand;
-        {
-            uint32_t o[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                rng.block(b, o);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int s = 4 * b + j;
-                    if (s < 15)
-                        rp[s] = u01(o[j]) - FLT_EPSILON;
-                    else
-                        u_depth_rand = u01(o[j]);
-                }
-            }
-        }
-        uint32_t done = 0, selmask = 0;
-        uint32_t wnib[4] = {0u, 0u, 0u, 0u};
-        float wsum = 0.f;
-        for (int v = 0; v < nv; ++v) {
-            const float cd = cdf[v * kBlock];
-            uint32_t hit = 0;
-#pragma unroll
-            for (int d = 0; d < 15; ++d) hit |= (uint32_t)(cd > rp[d]) << d;
-            const uint32_t fresh = hit & ~done;
-            done |= fresh;
-            const uint32_t cnt = __popc(fresh);
-            if (cnt) {
-                selmask |= 1u << v;
-                const uint32_t add = cnt << (4u * (uint32_t)(v & 7));
-                if ((v >> 3) == 0) wnib[0] |= add;
-                else if ((v >> 3) == 1) wnib[1] |= add;
-                else if ((v >> 3) == 2) wnib[2] |= add;
-                else wnib[3] |= add;
-                wsum += (float)cnt;
-            }
-        }
-
-        // ---- candidate final costs (src 983-1009) and the current
-        // hypothesis re-costed with the new view weights (its NCCs come from
-        // pm_candidates, row 8 of the record). Plain and prior passes walk
-        // the selected views once and read the view's 9 costs together; the
-        // geometric pass adds a reprojection term per (candidate, view).
-        // Every final cost is its weighted sum times one reciprocal of the
-        // weight sum: a candidate that is the current plane must compare
-        // equal to it (the adoption test is strict), and two divisions by
-        // wsum may be lowered to different approximate sequences.
-        float fc[8];
-        const float rws = 1.0f / wsum;
-        float4 cur = planes[c];
-        float cnow;
-        if (!P.geom) {
-            float sk[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) sk[k] = 0.f;
-            for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
-                const int v = __ffs(mm) - 1;
-                const float wv = nib_weight(wnib, v);
-                float xv[9];
-#pragma unroll
-                for (int k = 0; k < 9; ++k) xv[k] = ca_entry(ca, k, v, nv, NA, aidx);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) sk[k] = fmaf(wv, xv[k], sk[k]);
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) fc[k] = sk[k] * rws;
-            cnow = sk[8] * rws;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const bool fk = (flags >> k) & 1u;
-                float4 cp = make_float4(0.f, 0.f, 0.f, 1.f);
-                if (fk) cp = cplane[(size_t)k * NA + aidx];
-                float sk = 0.f;
-                for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
-                    const int v = __ffs(mm) - 1;
-                    const float wv = nib_weight(wnib, v);
-                    const float x = ca_entry(ca, k, v, nv, NA, aidx);
-                    const float g = fk ? x + 0.1f * geom_cost(rc, s_views[v], cp, fpx, fpy) : x;
-                    sk = fmaf(wv, g, sk);
-                }
-                fc[k] = sk * rws;
-            }
-            float sk = 0.f;
-            for (uint32_t mm = selmask; mm; mm &= mm - 1u) {
-                const int v = __ffs(mm) - 1;
-                const float x = ca_entry(ca, 8, v, nv, NA, aidx);
-                const float wv = nib_weight(wnib, v);
-                sk = fmaf(wv, x + 0.1f * geom_cost(rc, s_views[v], cur, fpx, fpy), sk);
-            }
-            cnow = sk * rws;
-        }
-        const int kmin = argmin_last8(fc);
-
-        float cost = cnow;
-        uint32_t selc = sel[c];
-        float depth_now = depth_from_plane(rc, cur, fpx, fpy);
-        const float4 cur0 = cur;
-
-        const bool masked = P.prior && P.plane_masks[c] > 0u;
-        float4 pp = make_float4(0.f, 0.f, 0.f, 0.f);
-        float dprior = 0.f;
-        if (P.prior) {
-            pp = P.prior_planes[c];
-            dprior = depth_from_plane(rc, pp, fpx, fpy);
-        }
-        float restricted = 0.f;
-        if (P.prior && masked) {
-            float rs[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                rs[k] = 0.f;
-                if ((flags >> k) & 1u) {
-                    const float4 cp = cplane[(size_t)k * NA + aidx];
-                    rs[k] = prior_score(P, fc[k], depth_from_plane(rc, cp, fpx, fpy), dprior, pp, cp);
-                }
-            }
-            const int kmax = argmax_last8(rs);
-            const float rnow = prior_score(P, cnow, depth_from_plane(rc, cur0, fpx, fpy), dprior, pp, cur0);
-            if ((flags >> kmax) & 1u) {
-                const float4 cp = cplane[(size_t)kmax * NA + aidx];
-                const float db = depth_from_plane(rc, cp, fpx, fpy);
-                if (db >= P.dmin && db <= P.dmax && rs[kmax] > rnow) {
-                    depth_now = db;
-                    cur = cp;
-                    cost = fc[kmax];
-                    restricted = rs[kmax];
-                    selc = selmask;
-                }
-            }
-        } else if ((flags >> kmin) & 1u) {
-            const float4 cp = cplane[(size_t)kmin * NA + aidx];
-            const float db = depth_from_plane(rc, cp, fpx, fpy);
-            if (db >= P.dmin && db <= P.dmax && fc[kmin] < cnow) {
-                depth_now = db;
-                cur = cp;
-                cost = fc[kmin];
-                // the prior path leaves it (lib 1306-1318) unless
-                // GMVS_COMPAT_FIX_PRIOR_SELECTION
-                if (!P.prior || (P.compat & GMVS_COMPAT_FIX_PRIOR_SELECTION)) selc = selmask;
-            }
-        }
-
-        // ---- refinement hypotheses (src 595-676, lib 684-811)
-        const float pert = 0.02f;
-        float drand;
-        float4 nrand;
-        uint32_t o4[4], o5[4];
-        rng.block(4, o4);  // slots 16..19
-        if (P.prior && masked) {
-            const float dsig = (P.dmax - P.dmin) / 64.0f;
-            drand = u_depth_rand * 6.0f * dsig + (dprior - 3.0f * dsig);
-            rng.block(5, o5);  // slots 20..23
-            nrand = perturbed_normal(rc, fpx, fpy, pp, u01(o5[0]), u01(o5[1]), u01(o5[2]),
-                                     kPiF * (5.0f / 180.0f));
-        } else {
-            drand = u_depth_rand * (P.dmax - P.dmin) + P.dmin;
-            nrand = random_normal(rc, fpx, fpy, rng, RNG_NORMAL, depth_now);
-        }
-        const float dlo = (1.0f - pert) * depth_now;
-        const float dhi = (1.0f + pert) * depth_now;
-        const float dpert = u01(o4[0]) * (dhi - dlo) + dlo;
-        const float4 npert = perturbed_normal(rc, fpx, fpy, cur, u01(o4[1]), u01(o4[2]), u01(o4[3]),
-                                              pert * kPiF);
-        const float dcur = depth_now;
-#pragma unroll
-        for (int i = 0; i < kRefine; ++i) {
-            const float di = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dcur);
-            float4 tp = (i == 0 || i == 4) ? cur : (i == 3 ? npert : nrand);
-            tp.w = dist_to_origin(rc, fpx, fpy, di, tp);
-            rtp[(size_t)i * NA + aidx] = tp;
-        }
-        planes[c] = cur;
-        costs[c] = cost;
-        sel[c] = selc;
-        rsw[(size_t)SW_SELMASK * NA + aidx] = selmask;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) rsw[(size_t)(SW_WNIB0 + i) * NA + aidx] = wnib[i];
-        rsw[(size_t)SW_RESTRICTED * NA + aidx] = __float_as_uint(restricted);
-        rsw[(size_t)SW_DRAND * NA + aidx] = __float_as_uint(drand);
-        rsw[(size_t)SW_DPERT * NA + aidx] = __float_as_uint(dpert);
-        rsw[(size_t)SW_DCUR * NA + aidx] = __float_as_uint(dcur);
-        if (P.geom) geom_count = ((uint32_t)__popc(flags) + 1u) * (uint32_t)__popc(selmask);
-    }
+    if (aidx < pr.na && a.ok)
+        geom_count = select_pixel(pr, color, iter, a, s_cdf + threadIdx.x, kBlock, s_views, false);
     if (work) count_work(work + 2 * K_SELECT, 0u, geom_count);
 }
 
-// Pixel `slot` of a pm_refine tile (slot = lane group in the NCC layout).
-__device__ inline Active active_slot(const PMConst& P, int tile, int color, int slot) {
-    const int TX = P.tile.tx(), TY = P.tile.ty();
-    const int tx = (P.wh + TX - 1) / TX;
-    const int ax = (tile % tx) * TX + (slot & (TX - 1));
-    const int y = (tile / tx) * TY + (slot >> P.tile.shift);
-    Active a;
-    a.sub = 0;
-    a.py = y;
-    a.px = 2 * ax + ((y + color) & 1);
-    a.aidx = y * P.wh + ax;
-    a.ok = (y < P.h) && (ax < P.wh) && (a.px < P.w);
-    return a;
-}
 
 template <int NT, int L>
 struct RefineSmem {
@@ -1161,7 +1212,8 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
                                       const float4* __restrict__ rtp,
                                       const uint32_t* __restrict__ rsw, int NA, RefineSmem<NT, L>& S,
                                       const ViewDesc* s_views, const Frame* s_fr, int ox, int oy,
-                                      uint16_t* units, float* res, uint32_t& ncc, uint32_t& geom) {
+                                      uint16_t* units, float* res, bool commit, uint32_t& ncc,
+                                      uint32_t& geom) {
     constexpr int NW = RefineSmem<NT, L>::NW;
     const RefCam& rc = P.rc;
     const int g = threadIdx.x / L, sub = threadIdx.x % L;
@@ -1274,7 +1326,9 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
             const float dpert = __uint_as_float(rec[(size_t)SW_DPERT * NA]);
             const float dcur = __uint_as_float(rec[(size_t)SW_DCUR * NA]);
             float restricted = __uint_as_float(rec[(size_t)SW_RESTRICTED * NA]);
-            float cost = costs[c];
+            // with the selection fused into pm_candidates the adopted state
+            // is still in the workspace: committed here either way
+            float cost = commit ? __uint_as_float(rec[(size_t)SW_COST * NA]) : costs[c];
             int best = -1;
 #pragma unroll 1
             for (int i = 0; i < kRefine; ++i) {
@@ -1302,6 +1356,9 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
             if (best >= 0) {
                 planes[c] = S.tp[best][sl];
                 costs[c] = cost;
+            } else if (commit) {
+                planes[c] = rtp[(size_t)kRefine * NA + a.aidx];
+                costs[c] = cost;
             }
         }
     }
@@ -1312,7 +1369,8 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
 template <int NT, int L, int PR>
 __global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(PR == PREC_F32 ? 3 : 4))) void pm_refine(const PMProblem* __restrict__ probs,
                                                    const WorkTables* __restrict__ tabs, int color,
-                                                   int nvmax, uint32_t* __restrict__ queue,
+                                                   int nvmax, int commit,
+                                                   uint32_t* __restrict__ queue,
                                                    unsigned long long* work) {
     __shared__ RefineSmem<NT, L> S;
     __shared__ Frame s_fr[kMaxViews];
@@ -1343,7 +1401,7 @@ __global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(PR
         const int ox = 2 * (tile % ntx) * TX, oy = (tile / ntx) * TY;
         tile_frames(pr.P, ox, oy, 2 * TX, TY, s_fr);  // the body's first barrier orders it
         pm_refine_body<NT, L, PR>(pr.P, tile, color, pr.planes, pr.costs, pr.rtp, pr.rsw, pr.na, S,
-                                  s_views, s_fr, ox, oy, units, res, ncc, geom);
+                                  s_views, s_fr, ox, oy, units, res, commit != 0, ncc, geom);
     }
     if (work) count_work(work + 2 * K_REFINE, ncc, geom);
 }
@@ -1550,7 +1608,7 @@ PMLayout pm_layout(int h, int w, int num_images, bool sweep, int precision) {
         L.cflag = o;
         o = align_up(o + na);
         L.rtp = o;
-        o = align_up(o + sizeof(float4) * kRefine * na);
+        o = align_up(o + sizeof(float4) * (kRefine + 1) * na);
         L.rsw = o;
         o = align_up(o + sizeof(uint32_t) * SW_NUM * na);
     } else {
@@ -1612,6 +1670,9 @@ int setup_views(int num_images, const gmvs_camera* cams, const float* const* ima
 
 // Clamp-free tap indexing allowed (gmvs_set_fast_taps; results do not depend on it).
 std::atomic<int> g_fast_taps{1};
+// View selection and adoption as pm_candidates' tile epilogue instead of the
+// pm_select launch (gmvs_set_select_fusion; results do not depend on it).
+std::atomic<int> g_fuse_select{0};
 
 PMConst make_const(int num_images, const gmvs_camera* cams, const float* ref, const ViewDesc* views,
                    const ViewF64* views64, const gmvs_params& prm, int radius) {
@@ -1851,7 +1912,9 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
     // lane-group width per kernel: block = 64 pixels x L lanes
     const int li = lanes_of(K_INIT, PR), lc = lanes_of(K_CANDIDATES, PR), lr = lanes_of(K_REFINE, PR);
     auto k_init = li == 2 ? pm_init<kNT, 2, PR> : pm_init<kNT, 4, PR>;
-    auto k_cand = lc == 2 ? pm_candidates<kNT, 2, PR> : pm_candidates<kNT, 4, PR>;
+    const bool fuse = g_fuse_select.load() != 0;
+    auto k_cand = fuse ? (lc == 2 ? pm_candidates<kNT, 2, PR, true> : pm_candidates<kNT, 4, PR, true>)
+                       : (lc == 2 ? pm_candidates<kNT, 2, PR, false> : pm_candidates<kNT, 4, PR, false>);
     auto k_ref = lr == 2 ? pm_refine<kNT, 2, PR> : pm_refine<kNT, 4, PR>;
     const int bi = kTilePix * li, bc = kTilePix * lc, br = kTilePix * lr;
     const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), bi, 0, n_full, K_INIT));
@@ -1866,13 +1929,15 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
         for (int color = 0; color < 2; ++color) {
             const int slot = 1 + 2 * (2 * it + color);
             GMVS_LAUNCH(K_CANDIDATES, st, k_cand, g_cand, dim3(bc), 0, st, d_probs, d_tabs, color,
-                        queue + kXcds * slot, work);
+                        it, queue + kXcds * slot, work);
             GMVS_LAUNCH_CHECK();
-            GMVS_LAUNCH(K_SELECT, st, pm_select, dim3(n_act), dim3(kBlock), sel_lds, st, d_probs, d_tabs,
-                        color, it, nvmax, work);
-            GMVS_LAUNCH_CHECK();
+            if (!fuse) {
+                GMVS_LAUNCH(K_SELECT, st, pm_select, dim3(n_act), dim3(kBlock), sel_lds, st, d_probs,
+                            d_tabs, color, it, nvmax, work);
+                GMVS_LAUNCH_CHECK();
+            }
             GMVS_LAUNCH(K_REFINE, st, k_ref, g_ref, dim3(br), ref_lds, st, d_probs, d_tabs, color,
-                        nvmax, queue + kXcds * (slot + 1), work);
+                        nvmax, fuse ? 1 : 0, queue + kXcds * (slot + 1), work);
             GMVS_LAUNCH_CHECK();
         }
     }
@@ -2014,6 +2079,13 @@ extern "C" int gmvs_set_fast_taps(int32_t enable) {
     clear_error();
     GMVS_ARG_CHECK(enable == 0 || enable == 1, "enable must be 0 or 1 (got %d)", enable);
     g_fast_taps.store(enable);
+    return GMVS_OK;
+}
+
+extern "C" int gmvs_set_select_fusion(int32_t enable) {
+    clear_error();
+    GMVS_ARG_CHECK(enable == 0 || enable == 1, "enable must be 0 or 1 (got %d)", enable);
+    g_fuse_select.store(enable);
     return GMVS_OK;
 }
 

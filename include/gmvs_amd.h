@@ -204,6 +204,14 @@ int gmvs_set_lane_width(int32_t kernel, int32_t lanes);
  * is set up. */
 int gmvs_set_fast_taps(int32_t enable);
 
+/* Kernel fusion (no counterpart in the reference): 1 runs the joint view
+ * selection and candidate adoption (gmvs/src/patch_match.cu:915-1040,
+ * gmvs/lib/src/patch_match_kernel.cu:1114-1334) as the epilogue of each
+ * candidate tile instead of as a separate launch; 0 (the default) launches
+ * them separately. Both give identical results. Process-wide, read when a
+ * run is set up. */
+int gmvs_set_select_fusion(int32_t enable);
+
 /* Parity hook: V bilateral NCCs for n (pixel, plane) pairs -- the batched form
  * of ComputeMultiViewCostVector (gmvs/lib/src/patch_match_kernel.cu:571-588,
  * gmvs/src/patch_match.cu:503-522). xy: device [n*2] int32, planes: device

@@ -660,6 +660,17 @@ def main() -> int:
         pts = fd_in.numel()
         fusion = {"mpts_per_s": round(pts / ft / 1e6, 1), "input_points": pts,
                   "fused_points": int((od > 0).sum().item()), "ms": round(ft * 1e3, 2)}
+        # the same fusion with the reprojection check in FP64
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        od64, _, _ = fuse_maps(fd_in, fn_in, fcams, fpairs, geom_consistent=3, reproj_px=1.0,
+                               rel_depth=0.01, angle_rad=math.radians(30.0), precision=64)
+        torch.cuda.synchronize()
+        ft64 = time.perf_counter() - t1
+        fusion["fp64_check"] = {"mpts_per_s": round(pts / ft64 / 1e6, 1),
+                                "fused_points": int((od64 > 0).sum().item()),
+                                "ms": round(ft64 * 1e3, 2)}
+        del od64
         log(f"fusion: {fusion}")
         fusion["post"] = post_fusion(fsc, od, on, not args.no_cpu_baseline)
         log(f"post-fusion gather and text: {fusion['post']}")

@@ -130,6 +130,7 @@ EXPORTS = (
     "gmvs_cost_batch",
     "gmvs_fuse_workspace_bytes",
     "gmvs_fuse_view",
+    "gmvs_fuse_view_fp64",
     "gmvs_fuse_noalloc_max_pixels",
     "gmvs_fuse_view_noalloc",
     "gmvs_gather_peak_buffer_bytes",
@@ -219,6 +220,7 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_fuse_view.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),
                                    ctypes.POINTER(vp), vp, i32, i32, vp, i32, f32, f32, f32, vp,
                                    vp, vp, sz, vp]
+    lib.gmvs_fuse_view_fp64.argtypes = lib.gmvs_fuse_view.argtypes
     lib.gmvs_fuse_noalloc_max_pixels.argtypes = []
     lib.gmvs_fuse_noalloc_max_pixels.restype = ctypes.c_int64
     lib.gmvs_fuse_view_noalloc.argtypes = [i32, ctypes.POINTER(CCamera), ctypes.POINTER(vp),

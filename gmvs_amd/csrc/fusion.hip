@@ -14,7 +14,9 @@
 //     written.
 // The angle test acos(dot) < theta is evaluated as dot > cos(theta), with a
 // dot outside [-1, 1] (or NaN) passing as acosf's NaN does in get_angle_cu
-// (fusion.cu:50-58).
+// (fusion.cu:50-58). gmvs_fuse_view_fp64 runs the same check with the world
+// points, projections and tests in FP64 (not in the reference), bit-exact
+// with the oracle's FP64 instantiation of the same loop.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -35,43 +37,52 @@ struct FuseView {
     int32_t global_id;
 };
 
+// The geometry is evaluated in T: float (the reference's FP32, the default)
+// or double (gmvs_fuse_view_fp64: the world points and projections in FP64,
+// for scenes whose world coordinates are large against their depths, where
+// FP32 world points carry reprojection errors near the 2 px threshold). The
+// camera parameters, depths and normals are FP32 inputs widened exactly.
+template <class T>
 struct F3 {
-    float x, y, z;
+    T x, y, z;
 };
 
-__device__ inline F3 fz_world_point(float x, float y, float depth, const gmvs_camera& c) {
-    F3 p;
-    p.x = depth * (x - c.K[2]) / c.K[0];
-    p.y = depth * (y - c.K[5]) / c.K[4];
+template <class T>
+__device__ inline F3<T> fz_world_point(T x, T y, T depth, const gmvs_camera& c) {
+    F3<T> p;
+    p.x = depth * (x - (T)c.K[2]) / (T)c.K[0];
+    p.y = depth * (y - (T)c.K[5]) / (T)c.K[4];
     p.z = depth;
-    F3 t;
-    t.x = c.R[0] * p.x + c.R[3] * p.y + c.R[6] * p.z;
-    t.y = c.R[1] * p.x + c.R[4] * p.y + c.R[7] * p.z;
-    t.z = c.R[2] * p.x + c.R[5] * p.y + c.R[8] * p.z;
-    F3 C;
-    C.x = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
-    C.y = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
-    C.z = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
-    F3 o;
+    F3<T> t;
+    t.x = (T)c.R[0] * p.x + (T)c.R[3] * p.y + (T)c.R[6] * p.z;
+    t.y = (T)c.R[1] * p.x + (T)c.R[4] * p.y + (T)c.R[7] * p.z;
+    t.z = (T)c.R[2] * p.x + (T)c.R[5] * p.y + (T)c.R[8] * p.z;
+    F3<T> C;
+    C.x = -((T)c.R[0] * (T)c.t[0] + (T)c.R[3] * (T)c.t[1] + (T)c.R[6] * (T)c.t[2]);
+    C.y = -((T)c.R[1] * (T)c.t[0] + (T)c.R[4] * (T)c.t[1] + (T)c.R[7] * (T)c.t[2]);
+    C.z = -((T)c.R[2] * (T)c.t[0] + (T)c.R[5] * (T)c.t[1] + (T)c.R[8] * (T)c.t[2]);
+    F3<T> o;
     o.x = t.x + C.x;
     o.y = t.y + C.y;
     o.z = t.z + C.z;
     return o;
 }
 
-__device__ inline void fz_project(const F3& X, const gmvs_camera& c, float& u, float& v, float& d) {
-    const float t0 = c.R[0] * X.x + c.R[1] * X.y + c.R[2] * X.z + c.t[0];
-    const float t1 = c.R[3] * X.x + c.R[4] * X.y + c.R[5] * X.z + c.t[1];
-    const float t2 = c.R[6] * X.x + c.R[7] * X.y + c.R[8] * X.z + c.t[2];
-    d = c.K[6] * t0 + c.K[7] * t1 + c.K[8] * t2;
-    u = (c.K[0] * t0 + c.K[1] * t1 + c.K[2] * t2) / d;
-    v = (c.K[3] * t0 + c.K[4] * t1 + c.K[5] * t2) / d;
+template <class T>
+__device__ inline void fz_project(const F3<T>& X, const gmvs_camera& c, T& u, T& v, T& d) {
+    const T t0 = (T)c.R[0] * X.x + (T)c.R[1] * X.y + (T)c.R[2] * X.z + (T)c.t[0];
+    const T t1 = (T)c.R[3] * X.x + (T)c.R[4] * X.y + (T)c.R[5] * X.z + (T)c.t[1];
+    const T t2 = (T)c.R[6] * X.x + (T)c.R[7] * X.y + (T)c.R[8] * X.z + (T)c.t[2];
+    d = (T)c.K[6] * t0 + (T)c.K[7] * t1 + (T)c.K[8] * t2;
+    u = ((T)c.K[0] * t0 + (T)c.K[1] * t1 + (T)c.K[2] * t2) / d;
+    v = ((T)c.K[3] * t0 + (T)c.K[4] * t1 + (T)c.K[5] * t2) / d;
 }
 
-// int(x + 0.5f) in [0, n): truncation toward zero (fusion.cu:112-113).
-__device__ inline bool fz_round_in(float x, int n, int* out) {
-    const float c = x + 0.5f;
-    if (!(c > -1.0f && c < (float)n)) return false;
+// int(x + 0.5) in [0, n): truncation toward zero (fusion.cu:112-113).
+template <class T>
+__device__ inline bool fz_round_in(T x, int n, int* out) {
+    const T c = x + (T)0.5;
+    if (!(c > (T)-1 && c < (T)n)) return false;
     *out = (int)c;
     return true;
 }
@@ -95,6 +106,7 @@ __global__ void fuse_fill(int hw, float* __restrict__ depth, float* __restrict__
 
 // Pass 1: per reference pixel, the consistency check against every source,
 // the averaged point/normal, its reprojection and the mask updates.
+template <class T>
 __global__ __launch_bounds__(256) void fuse_check(const FuseView* __restrict__ views, int nsrc,
                                                   int h, int w, int geom_consistent,
                                                   float reproj_px, float rel_depth,
@@ -109,34 +121,35 @@ __global__ __launch_bounds__(256) void fuse_check(const FuseView* __restrict__ v
     rec_idx[ri] = -1;
     if (ref_mask[ri]) return;
     const gmvs_camera& rc = views[0].cam;
-    const float rd = views[0].depth[ri];
-    const F3 rn{views[0].normal[3 * ri], views[0].normal[3 * ri + 1], views[0].normal[3 * ri + 2]};
-    if (rd <= 0.0f) return;
+    const T rd = views[0].depth[ri];
+    const F3<T> rn{views[0].normal[3 * ri], views[0].normal[3 * ri + 1],
+                   views[0].normal[3 * ri + 2]};
+    if (rd <= (T)0) return;
     const int r = ri / w, cc = ri - r * w;
-    const F3 rp = fz_world_point((float)cc, (float)r, rd, rc);
-    F3 cp = rp, cn = rn;
+    const F3<T> rp = fz_world_point((T)cc, (T)r, rd, rc);
+    F3<T> cp = rp, cn = rn;
     int num = 0;
     uint32_t used_mask = 0;
     for (int j = 0; j < nsrc; ++j) {
         const FuseView& sv = views[j + 1];
-        float u, v, pd;
+        T u, v, pd;
         fz_project(rp, sv.cam, u, v, pd);
         int sr, scol;
         if (!fz_round_in(v, h, &sr) || !fz_round_in(u, w, &scol)) continue;
         const int si = sr * w + scol;
-        const float sd = sv.depth[si];
-        const F3 sn{sv.normal[3 * si], sv.normal[3 * si + 1], sv.normal[3 * si + 2]};
-        if (sd <= 0.0f) continue;
-        const F3 sp = fz_world_point((float)scol, (float)sr, sd, sv.cam);
-        float tu, tv;
+        const T sd = sv.depth[si];
+        const F3<T> sn{sv.normal[3 * si], sv.normal[3 * si + 1], sv.normal[3 * si + 2]};
+        if (sd <= (T)0) continue;
+        const F3<T> sp = fz_world_point((T)scol, (T)sr, sd, sv.cam);
+        T tu, tv;
         fz_project(sp, rc, tu, tv, pd);
-        const float dx = (float)cc - tu;
-        const float dy = (float)r - tv;
-        const float err = sqrtf(dx * dx + dy * dy);
-        const float rel = fabsf(pd - rd) / rd;
-        const float dot = rn.x * sn.x + rn.y * sn.y + rn.z * sn.z;
-        const bool ang_ok = !(dot >= -1.0f && dot <= 1.0f) || dot > cos_angle;
-        if (err < reproj_px && rel < rel_depth && ang_ok) {
+        const T dx = (T)cc - tu;
+        const T dy = (T)r - tv;
+        const T err = sqrt(dx * dx + dy * dy);
+        const T rel = fabs(pd - rd) / rd;
+        const T dot = rn.x * sn.x + rn.y * sn.y + rn.z * sn.z;
+        const bool ang_ok = !(dot >= (T)-1 && dot <= (T)1) || dot > (T)cos_angle;
+        if (err < (T)reproj_px && rel < (T)rel_depth && ang_ok) {
             cp.x += sp.x;
             cp.y += sp.y;
             cp.z += sp.z;
@@ -148,28 +161,28 @@ __global__ __launch_bounds__(256) void fuse_check(const FuseView* __restrict__ v
         }
     }
     if (num < geom_consistent) return;
-    const float den = (float)num + 1.0f;
+    const T den = (T)num + (T)1;
     cp.x /= den;
     cp.y /= den;
     cp.z /= den;
     cn.x /= den;
     cn.y /= den;
     cn.z /= den;
-    float pu, pv, pdep;
+    T pu, pv, pdep;
     fz_project(cp, rc, pu, pv, pdep);
     int pr, pc;
-    if (fz_round_in(pv, h, &pr) && fz_round_in(pu, w, &pc) && pdep > 0.001f) {
+    if (fz_round_in(pv, h, &pr) && fz_round_in(pu, w, &pc) && pdep > (T)0.001f) {
         const int pi = pr * w + pc;
         atomicMax(&winner[pi], ri);
         rec_idx[ri] = pi;
-        rec_val[ri] = make_float4(pdep, cn.x, cn.y, cn.z);
+        rec_val[ri] = make_float4((float)pdep, (float)cn.x, (float)cn.y, (float)cn.z);
     }
     // mark the used source pixels (recomputed: the same arithmetic as above)
     const size_t hw = (size_t)h * w;
     for (int j = 0; j < nsrc; ++j) {
         if (!((used_mask >> j) & 1u)) continue;
         const FuseView& sv = views[j + 1];
-        float u, v, pd;
+        T u, v, pd;
         fz_project(rp, sv.cam, u, v, pd);
         int sr, scol;
         fz_round_in(v, h, &sr);
@@ -227,14 +240,15 @@ extern "C" size_t gmvs_fuse_workspace_bytes(int32_t height, int32_t width) {
     return fuse_layout(size_t(height) * width).total;
 }
 
-extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
-                              const float* const* depths, const float* const* normals,
-                              uint8_t* masks, int32_t num_mask_views, int32_t ref_id,
-                              const int32_t* src_ids,
-                              int32_t geom_consistent, float reproj_px, float rel_depth,
-                              float cos_angle, float* out_depth, float* out_normal,
-                              void* workspace, size_t workspace_bytes, void* stream) {
-    clear_error();
+namespace gmvs {
+namespace {
+
+int fuse_view_impl(bool fp64, int32_t num_images, const gmvs_camera* cameras,
+                   const float* const* depths, const float* const* normals, uint8_t* masks,
+                   int32_t num_mask_views, int32_t ref_id, const int32_t* src_ids,
+                   int32_t geom_consistent, float reproj_px, float rel_depth, float cos_angle,
+                   float* out_depth, float* out_normal, void* workspace, size_t workspace_bytes,
+                   void* stream) {
     GMVS_ARG_CHECK(cameras && depths && normals && masks && out_depth && out_normal,
                    "fuse_view: NULL argument");
     GMVS_ARG_CHECK(num_images >= 1 && num_images <= GMVS_MAX_NGB + 1,
@@ -282,7 +296,8 @@ extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
     GMVS_LAUNCH(K_FUSE_FILL, st, fuse_fill, grid, block, 0, st, hw, out_depth, out_normal, winner,
                        masks + (size_t)ref_id * hw, mask_copy);
     GMVS_LAUNCH_CHECK();
-    GMVS_LAUNCH(K_FUSE_CHECK, st, fuse_check, grid, block, 0, st,
+    auto check = fp64 ? fuse_check<double> : fuse_check<float>;
+    GMVS_LAUNCH(K_FUSE_CHECK, st, check, grid, block, 0, st,
                        reinterpret_cast<const FuseView*>(ws + L.views), num_images - 1, h, w,
                        geom_consistent, reproj_px, rel_depth, cos_angle, mask_copy, masks, winner,
                        rec_idx, rec_val);
@@ -291,6 +306,35 @@ extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
                        out_normal);
     GMVS_LAUNCH_CHECK();
     return GMVS_OK;
+}
+
+}  // namespace
+}  // namespace gmvs
+
+extern "C" int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras,
+                              const float* const* depths, const float* const* normals,
+                              uint8_t* masks, int32_t num_mask_views, int32_t ref_id,
+                              const int32_t* src_ids,
+                              int32_t geom_consistent, float reproj_px, float rel_depth,
+                              float cos_angle, float* out_depth, float* out_normal,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+    clear_error();
+    return fuse_view_impl(false, num_images, cameras, depths, normals, masks, num_mask_views,
+                          ref_id, src_ids, geom_consistent, reproj_px, rel_depth, cos_angle,
+                          out_depth, out_normal, workspace, workspace_bytes, stream);
+}
+
+extern "C" int gmvs_fuse_view_fp64(int32_t num_images, const gmvs_camera* cameras,
+                                   const float* const* depths, const float* const* normals,
+                                   uint8_t* masks, int32_t num_mask_views, int32_t ref_id,
+                                   const int32_t* src_ids, int32_t geom_consistent,
+                                   float reproj_px, float rel_depth, float cos_angle,
+                                   float* out_depth, float* out_normal, void* workspace,
+                                   size_t workspace_bytes, void* stream) {
+    clear_error();
+    return fuse_view_impl(true, num_images, cameras, depths, normals, masks, num_mask_views,
+                          ref_id, src_ids, geom_consistent, reproj_px, rel_depth, cos_angle,
+                          out_depth, out_normal, workspace, workspace_bytes, stream);
 }
 
 // ---- no-alloc variant ------------------------------------------------------

@@ -274,9 +274,10 @@ class MultiViewPipeline:
 def fuse_maps(depths: torch.Tensor, normals: torch.Tensor, cameras: Sequence[S.Camera],
               pairs: Sequence[Sequence[int]], geom_consistent: int = 3,
               reproj_px: float = S.FUSE_REPROJ_PX, rel_depth: float = S.FUSE_REL_DEPTH,
-              angle_rad: float = S.FUSE_ANGLE_RAD, workspace: Optional[S._Workspace] = None
-              ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
-    """Fusion of device-resident depth/normal maps of all views (in view order)."""
+              angle_rad: float = S.FUSE_ANGLE_RAD, workspace: Optional[S._Workspace] = None,
+              precision: int = 32) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Fusion of device-resident depth/normal maps of all views (in view order);
+    precision 64 runs the reprojection check in FP64 (gmvs_fuse_view_fp64)."""
     n, h, w = (int(x) for x in depths.shape)
     dev = depths.device
     ws = workspace or S._Workspace()
@@ -287,7 +288,7 @@ def fuse_maps(depths: torch.Tensor, normals: torch.Tensor, cameras: Sequence[S.C
         ids = [ref] + list(pairs[ref])
         S.fuse_view(ids, [cameras[i]._c for i in ids], [depths[i] for i in ids],
                     [normals[i] for i in ids], masks, geom_consistent, out_d[ref], out_n[ref],
-                    reproj_px, rel_depth, angle_rad, ws)
+                    reproj_px, rel_depth, angle_rad, ws, precision=precision)
     return out_d, out_n, masks
 
 

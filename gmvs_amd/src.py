@@ -390,6 +390,9 @@ class Fuser:
         self.reproj_px = FUSE_REPROJ_PX
         self.rel_depth = FUSE_REL_DEPTH
         self.angle_rad = FUSE_ANGLE_RAD
+        # 32: the reference's FP32 reprojection check; 64: the check in FP64
+        # (gmvs_fuse_view_fp64), for world coordinates large against depths
+        self.precision = 32
 
     def load_samples(self, problems: Sequence[Problem], cameras: Sequence[Camera],
                      depths: torch.Tensor, normals: torch.Tensor, masks: torch.Tensor) -> None:
@@ -425,7 +428,8 @@ class Fuser:
         out_n = torch.empty(h, w, 3, dtype=torch.float32, device=dev)
         fuse_view(ids, [self._cameras[i]._c for i in ids], [depths[i] for i in ids],
                   [normals[i] for i in ids], self.masks_cuda, int(geom_consistent), out_d, out_n,
-                  self.reproj_px, self.rel_depth, self.angle_rad, self._ws)
+                  self.reproj_px, self.rel_depth, self.angle_rad, self._ws,
+                  precision=self.precision)
         return out_d, out_n
 
 
@@ -433,14 +437,19 @@ def fuse_view(ids: Sequence[int], cams: Sequence[N.CCamera], depths: Sequence[to
               normals: Sequence[torch.Tensor], masks: torch.Tensor, geom_consistent: int,
               out_depth: torch.Tensor, out_normal: torch.Tensor,
               reproj_px: float = FUSE_REPROJ_PX, rel_depth: float = FUSE_REL_DEPTH,
-              angle_rad: float = FUSE_ANGLE_RAD, workspace: Optional[_Workspace] = None) -> None:
-    """Thin call of gmvs_fuse_view on device tensors."""
+              angle_rad: float = FUSE_ANGLE_RAD, workspace: Optional[_Workspace] = None,
+              precision: int = 32) -> None:
+    """Thin call of gmvs_fuse_view (precision 32) or gmvs_fuse_view_fp64
+    (precision 64) on device tensors."""
+    if precision not in (32, 64):
+        raise ValueError(f"fusion precision must be 32 or 64 (got {precision!r})")
     h, w = out_depth.shape
     ws = (workspace or _Workspace()).get(N.lib.gmvs_fuse_workspace_bytes(h, w), out_depth.device)
     cam_arr = N.camera_array(cams)
     src = (ctypes.c_int32 * max(1, len(ids) - 1))(*ids[1:])
     cos_t = float(np.float32(math.cos(angle_rad)))
-    N.check("gmvs_fuse_view", N.lib.gmvs_fuse_view(
+    name = "gmvs_fuse_view" if precision == 32 else "gmvs_fuse_view_fp64"
+    N.check(name, getattr(N.lib, name)(
         len(ids), ctypes.cast(cam_arr, ctypes.POINTER(N.CCamera)),
         N.ptr_array([d.data_ptr() for d in depths]), N.ptr_array([n.data_ptr() for n in normals]),
         ctypes.c_void_p(masks.data_ptr()), int(masks.shape[0]), int(ids[0]), src,

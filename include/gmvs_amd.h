@@ -238,6 +238,21 @@ int gmvs_fuse_view(int32_t num_images, const gmvs_camera* cameras, const float* 
                    float rel_depth, float cos_angle, float* out_depth, float* out_normal,
                    void* workspace, size_t workspace_bytes, void* stream);
 
+/* gmvs_fuse_view with the reprojection check in FP64 (not in the reference,
+ * whose batch_fusion_kernel is FP32: gmvs/src/fusion.cu:61-182): the world
+ * points, both projections, the reprojection error, the relative depth
+ * difference and the averaged point in double; inputs, thresholds and
+ * outputs as gmvs_fuse_view. For scenes whose world coordinates are large
+ * against their depths (geo-referenced or far-from-origin reconstructions),
+ * where FP32 world points carry reprojection errors near the 2 px
+ * threshold. Same workspace (gmvs_fuse_workspace_bytes). */
+int gmvs_fuse_view_fp64(int32_t num_images, const gmvs_camera* cameras,
+                        const float* const* depths, const float* const* normals, uint8_t* masks,
+                        int32_t num_mask_views, int32_t ref_id, const int32_t* src_ids,
+                        int32_t geom_consistent, float reproj_px, float rel_depth,
+                        float cos_angle, float* out_depth, float* out_normal, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
 /* No counterpart in the reference: gmvs_fuse_view for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc). The same arguments without the workspace:

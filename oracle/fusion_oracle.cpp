@@ -19,6 +19,10 @@
 //     what the in-order write phase's last writer is (fusion.cu:164-171);
 //   * masks[ref] is read for the whole view before any mask of this view is
 //     written (the reference only writes masks of source views).
+//
+// The loop is written once over the geometry type T: float is the
+// reference's FP32 (oracle_fuse_view), double the FP64 check of
+// gmvs_fuse_view_fp64 (oracle_fuse_view64), the inputs widened exactly.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -30,60 +34,58 @@
 
 namespace {
 
+template <class T>
 struct P3 {
-    float x, y, z;
+    T x, y, z;
 };
 
-P3 world_point(float x, float y, float depth, const OCamera& c) {
-    P3 p;
-    p.x = depth * (x - c.K[2]) / c.K[0];
-    p.y = depth * (y - c.K[5]) / c.K[4];
+// get_3D_point_on_world_cu (fusion.cu:26-40)
+template <class T>
+P3<T> world_point(T x, T y, T depth, const OCamera& c) {
+    P3<T> p;
+    p.x = depth * (x - T(c.K[2])) / T(c.K[0]);
+    p.y = depth * (y - T(c.K[5])) / T(c.K[4]);
     p.z = depth;
-    P3 t;
-    t.x = c.R[0] * p.x + c.R[3] * p.y + c.R[6] * p.z;
-    t.y = c.R[1] * p.x + c.R[4] * p.y + c.R[7] * p.z;
-    t.z = c.R[2] * p.x + c.R[5] * p.y + c.R[8] * p.z;
-    P3 C;
-    C.x = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
-    C.y = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
-    C.z = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
-    P3 o;
+    P3<T> t;
+    t.x = T(c.R[0]) * p.x + T(c.R[3]) * p.y + T(c.R[6]) * p.z;
+    t.y = T(c.R[1]) * p.x + T(c.R[4]) * p.y + T(c.R[7]) * p.z;
+    t.z = T(c.R[2]) * p.x + T(c.R[5]) * p.y + T(c.R[8]) * p.z;
+    P3<T> C;
+    C.x = -(T(c.R[0]) * T(c.t[0]) + T(c.R[3]) * T(c.t[1]) + T(c.R[6]) * T(c.t[2]));
+    C.y = -(T(c.R[1]) * T(c.t[0]) + T(c.R[4]) * T(c.t[1]) + T(c.R[7]) * T(c.t[2]));
+    C.z = -(T(c.R[2]) * T(c.t[0]) + T(c.R[5]) * T(c.t[1]) + T(c.R[8]) * T(c.t[2]));
+    P3<T> o;
     o.x = t.x + C.x;
     o.y = t.y + C.y;
     o.z = t.z + C.z;
     return o;
 }
 
-void project(const P3& X, const OCamera& c, float& u, float& v, float& d) {
-    const float t0 = c.R[0] * X.x + c.R[1] * X.y + c.R[2] * X.z + c.t[0];
-    const float t1 = c.R[3] * X.x + c.R[4] * X.y + c.R[5] * X.z + c.t[1];
-    const float t2 = c.R[6] * X.x + c.R[7] * X.y + c.R[8] * X.z + c.t[2];
-    d = c.K[6] * t0 + c.K[7] * t1 + c.K[8] * t2;
-    u = (c.K[0] * t0 + c.K[1] * t1 + c.K[2] * t2) / d;
-    v = (c.K[3] * t0 + c.K[4] * t1 + c.K[5] * t2) / d;
+// project_on_camera_cu (fusion.cu:42-48)
+template <class T>
+void project(const P3<T>& X, const OCamera& c, T& u, T& v, T& d) {
+    const T t0 = T(c.R[0]) * X.x + T(c.R[1]) * X.y + T(c.R[2]) * X.z + T(c.t[0]);
+    const T t1 = T(c.R[3]) * X.x + T(c.R[4]) * X.y + T(c.R[5]) * X.z + T(c.t[1]);
+    const T t2 = T(c.R[6]) * X.x + T(c.R[7]) * X.y + T(c.R[8]) * X.z + T(c.t[2]);
+    d = T(c.K[6]) * t0 + T(c.K[7]) * t1 + T(c.K[8]) * t2;
+    u = (T(c.K[0]) * t0 + T(c.K[1]) * t1 + T(c.K[2]) * t2) / d;
+    v = (T(c.K[3]) * t0 + T(c.K[4]) * t1 + T(c.K[5]) * t2) / d;
 }
 
-// int(x + 0.5f) in [0, n) ?
-bool round_in(float x, int n, int* out) {
-    const float c = x + 0.5f;
-    if (!(c > -1.0f && c < float(n))) return false;
+// int(x + 0.5) in [0, n) ?
+template <class T>
+bool round_in(T x, int n, int* out) {
+    const T c = x + T(0.5);
+    if (!(c > T(-1) && c < T(n))) return false;
     *out = int(c);
     return true;
 }
 
-}  // namespace
-
-extern "C" {
-
-// Fuses one reference view. cams/depths/normals hold the reference first and
-// then its num_images-1 sources; masks is the global [N, H, W] byte mask,
-// indexed with ref_id and src_ids (global view ids). out_depth [H*W] and
-// out_normal [H*W*3] must be zeroed by the caller. Returns the number of
-// consistent reference pixels (before the scatter).
-int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const* depths,
-                         const float* const* normals, uint8_t* masks, int ref_id,
-                         const int32_t* src_ids, int geom_consistent, float reproj_px,
-                         float rel_depth, float cos_angle, float* out_depth, float* out_normal) {
+template <class T>
+int64_t fuse_view(const OCamera* cams, int num_images, const float* const* depths,
+                  const float* const* normals, uint8_t* masks, int ref_id, const int32_t* src_ids,
+                  int geom_consistent, float reproj_px, float rel_depth, float cos_angle,
+                  float* out_depth, float* out_normal) {
     const int h = cams[0].height, w = cams[0].width;
     const size_t hw = size_t(h) * w;
     const int nsrc = num_images - 1;
@@ -106,35 +108,35 @@ int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const
             const size_t ri = size_t(r) * w + cc;
             int16_t* u_ri = &used[ri * size_t(nsrc) * 2];
             if (ref_mask[ri]) continue;
-            const float rd = depths[0][ri];
-            const P3 rn{normals[0][3 * ri], normals[0][3 * ri + 1], normals[0][3 * ri + 2]};
-            if (rd <= 0.0f) continue;
-            const P3 rp = world_point(float(cc), float(r), rd, rc);
-            P3 cp = rp, cn = rn;
+            const T rd = depths[0][ri];
+            const P3<T> rn{normals[0][3 * ri], normals[0][3 * ri + 1], normals[0][3 * ri + 2]};
+            if (rd <= T(0)) continue;
+            const P3<T> rp = world_point(T(cc), T(r), rd, rc);
+            P3<T> cp = rp, cn = rn;
             int num = 0;
             for (int j = 0; j < nsrc; ++j) {
                 u_ri[2 * j] = -1;
                 u_ri[2 * j + 1] = -1;
                 const OCamera& sc = cams[j + 1];
-                float u, v, pd;
+                T u, v, pd;
                 project(rp, sc, u, v, pd);
                 int sr, scol;
                 if (!round_in(v, h, &sr) || !round_in(u, w, &scol)) continue;
                 const size_t si = size_t(sr) * w + scol;
-                const float sd = depths[j + 1][si];
-                const P3 sn{normals[j + 1][3 * si], normals[j + 1][3 * si + 1],
-                            normals[j + 1][3 * si + 2]};
-                if (sd <= 0.0f) continue;
-                const P3 sp = world_point(float(scol), float(sr), sd, sc);
-                float tu, tv;
+                const T sd = depths[j + 1][si];
+                const P3<T> sn{normals[j + 1][3 * si], normals[j + 1][3 * si + 1],
+                               normals[j + 1][3 * si + 2]};
+                if (sd <= T(0)) continue;
+                const P3<T> sp = world_point(T(scol), T(sr), sd, sc);
+                T tu, tv;
                 project(sp, rc, tu, tv, pd);
-                const float dx = float(cc) - tu;
-                const float dy = float(r) - tv;
-                const float err = std::sqrt(dx * dx + dy * dy);
-                const float rel = std::fabs(pd - rd) / rd;
-                const float dot = rn.x * sn.x + rn.y * sn.y + rn.z * sn.z;
-                const bool ang_ok = !(dot >= -1.0f && dot <= 1.0f) || dot > cos_angle;
-                if (err < reproj_px && rel < rel_depth && ang_ok) {
+                const T dx = T(cc) - tu;
+                const T dy = T(r) - tv;
+                const T err = std::sqrt(dx * dx + dy * dy);
+                const T rel = std::fabs(pd - rd) / rd;
+                const T dot = rn.x * sn.x + rn.y * sn.y + rn.z * sn.z;
+                const bool ang_ok = !(dot >= T(-1) && dot <= T(1)) || dot > T(cos_angle);
+                if (err < T(reproj_px) && rel < T(rel_depth) && ang_ok) {
                     cp.x += sp.x;
                     cp.y += sp.y;
                     cp.z += sp.z;
@@ -148,22 +150,22 @@ int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const
             }
             if (num < geom_consistent) continue;
             ok[ri] = 1;
-            const float den = float(num) + 1.0f;
+            const T den = T(num) + T(1);
             cp.x /= den;
             cp.y /= den;
             cp.z /= den;
             cn.x /= den;
             cn.y /= den;
             cn.z /= den;
-            float pu, pv, pdp;
+            T pu, pv, pdp;
             project(cp, rc, pu, pv, pdp);
             int pr, pc;
-            if (round_in(pv, h, &pr) && round_in(pu, w, &pc) && pdp > 0.001f) {
+            if (round_in(pv, h, &pr) && round_in(pu, w, &pc) && pdp > T(0.001f)) {
                 dest[ri] = int32_t(size_t(pr) * w + pc);
-                pdep[ri] = pdp;
-                pnrm[3 * ri] = cn.x;
-                pnrm[3 * ri + 1] = cn.y;
-                pnrm[3 * ri + 2] = cn.z;
+                pdep[ri] = float(pdp);
+                pnrm[3 * ri] = float(cn.x);
+                pnrm[3 * ri + 1] = float(cn.y);
+                pnrm[3 * ri + 2] = float(cn.z);
             }
         }
     }
@@ -185,6 +187,34 @@ int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const
         }
     }
     return consistent_pixels;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Fuses one reference view. cams/depths/normals hold the reference first and
+// then its num_images-1 sources; masks is the global [N, H, W] byte mask,
+// indexed with ref_id and src_ids (global view ids). out_depth [H*W] and
+// out_normal [H*W*3] must be zeroed by the caller. Returns the number of
+// consistent reference pixels (before the scatter).
+int64_t oracle_fuse_view(const OCamera* cams, int num_images, const float* const* depths,
+                         const float* const* normals, uint8_t* masks, int ref_id,
+                         const int32_t* src_ids, int geom_consistent, float reproj_px,
+                         float rel_depth, float cos_angle, float* out_depth, float* out_normal) {
+    return fuse_view<float>(cams, num_images, depths, normals, masks, ref_id, src_ids,
+                            geom_consistent, reproj_px, rel_depth, cos_angle, out_depth,
+                            out_normal);
+}
+
+// The same with the geometry in FP64 (gmvs_fuse_view_fp64).
+int64_t oracle_fuse_view64(const OCamera* cams, int num_images, const float* const* depths,
+                           const float* const* normals, uint8_t* masks, int ref_id,
+                           const int32_t* src_ids, int geom_consistent, float reproj_px,
+                           float rel_depth, float cos_angle, float* out_depth, float* out_normal) {
+    return fuse_view<double>(cams, num_images, depths, normals, masks, ref_id, src_ids,
+                             geom_consistent, reproj_px, rel_depth, cos_angle, out_depth,
+                             out_normal);
 }
 
 }  // extern "C"

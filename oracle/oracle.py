@@ -92,6 +92,8 @@ def _load():
                                      ctypes.POINTER(vp), vp, ctypes.c_int, vp, ctypes.c_int,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
     lib.oracle_fuse_view.restype = ctypes.c_int64
+    lib.oracle_fuse_view64.argtypes = lib.oracle_fuse_view.argtypes
+    lib.oracle_fuse_view64.restype = ctypes.c_int64
     i32, f32 = ctypes.c_int, ctypes.c_float
     lib.oracle_sliding_window_normal.argtypes = [vp, vp, i32, i32, i32, vp]
     lib.oracle_sliding_window_normal.restype = None
@@ -282,8 +284,9 @@ def pm_lockstep(cams, images, params, cap: int = 1 << 20, **kw):
 def fuse_view(cams: Sequence[OCamera], depths: Sequence[np.ndarray],
               normals: Sequence[np.ndarray], masks: np.ndarray, ref_id: int,
               src_ids: Sequence[int], geom_consistent: int, reproj_px: float,
-              rel_depth: float, cos_angle: float):
-    """Sequential FP32 fusion of one view; masks ([N,H,W] uint8) is updated in place."""
+              rel_depth: float, cos_angle: float, precision: int = 32):
+    """Sequential fusion of one view, the geometry in FP32 (the reference) or
+    FP64 (precision 64); masks ([N,H,W] uint8) is updated in place."""
     ds = [_f32(a) for a in depths]
     ns = [_f32(a) for a in normals]
     h, w = ds[0].shape
@@ -291,9 +294,9 @@ def fuse_view(cams: Sequence[OCamera], depths: Sequence[np.ndarray],
     on = np.zeros((h, w, 3), np.float32)
     sids = np.ascontiguousarray(src_ids, np.int32)
     assert masks.dtype == np.uint8 and masks.flags.c_contiguous
-    n = lib().oracle_fuse_view(_cams(cams), len(ds), _ptrs(ds), _ptrs(ns), _ptr(masks), ref_id,
-                               _ptr(sids), geom_consistent, reproj_px, rel_depth, cos_angle,
-                               _ptr(od), _ptr(on))
+    fn = lib().oracle_fuse_view if precision == 32 else lib().oracle_fuse_view64
+    n = fn(_cams(cams), len(ds), _ptrs(ds), _ptrs(ns), _ptr(masks), ref_id, _ptr(sids),
+           geom_consistent, reproj_px, rel_depth, cos_angle, _ptr(od), _ptr(on))
     return od, on, int(n)
 
 

@@ -6,8 +6,16 @@ cameras and depths chosen so every intermediate value is a short binary
 fraction (exact in FP32): consistency, the count + 1 average, the rounding
 int(x + 0.5f) with truncation toward zero, the strict threshold tests, the
 last writer of a scatter collision and the mask side effects across views.
+Every case runs on both instantiations of the twin: FP32 (the reference's
+arithmetic) and FP64 (the check of gmvs_fuse_view_fp64); the values are exact
+in both, so both must give the closed-form answers. A last test pins the
+FP64 instantiation by an invariance of the exact problem: a change of length
+unit (depths and camera translations x 1000) leaves every decision unchanged.
 """
+import math
+
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 
@@ -34,9 +42,20 @@ def _normals(fn):
 NZ = _normals(lambda c, r: (0.0, 0.0, -1.0))
 
 
+PREC = [32]
+
+
+@pytest.fixture(autouse=True, params=[32, 64])
+def precision(request):
+    PREC[0] = request.param
+    yield request.param
+    PREC[0] = 32
+
+
 def _fuse(cams, depths, normals, masks, ref_id, src_ids, geom=1, reproj=2.0, rel=0.01,
           cos_angle=-2.0):
-    return O.fuse_view(cams, depths, normals, masks, ref_id, src_ids, geom, reproj, rel, cos_angle)
+    return O.fuse_view(cams, depths, normals, masks, ref_id, src_ids, geom, reproj, rel, cos_angle,
+                       precision=PREC[0])
 
 
 def _masks(n=3):
@@ -159,3 +178,47 @@ def test_masks_carry_across_views():
     # in the other order view 1 fuses everything
     m2 = _masks(2)
     assert _fuse(cams, depths, [NZ, NZ], m2, 1, [0])[2] == H * W and m2[0].all()
+
+
+def test_fp64_check_closer_to_unit_invariance_than_fp32(precision):
+    """Depths and camera translations x 1000 (FP32 inputs, each rounded once):
+    exact arithmetic gives the same fusion, so every decision that changes is
+    a rounding artefact. Measured on this scene (~35,000 fused pixels): the
+    FP32 check changes 8 source-mask bits and 4 fused pixels, the FP64 check
+    no mask bit and 2 fused pixels (decisions within the inputs' own FP32
+    rounding of a threshold, which no arithmetic precision removes)."""
+    if precision != 64:
+        pytest.skip("compares the two instantiations once")
+    from tests import helpers as T
+    from tests.test_gpu_fusion import noisy_maps
+
+    sc = T.scene()
+    n = sc.depth.shape[0]
+    ds, ns = noisy_maps(sc, np.random.default_rng(7))
+    h, w = ds[0].shape
+    cos_t = float(np.float32(math.cos(0.174533)))
+
+    def run(scale, prec):
+        cams = [O.make_camera(c.K, c.R, (np.asarray(c.t, np.float64) * scale).astype(np.float32),
+                              c.height, c.width, c.depth_min * scale, c.depth_max * scale)
+                for c in sc.cameras]
+        dsc = [(d.astype(np.float64) * scale).astype(np.float32) for d in ds]
+        masks = np.zeros((n, h, w), np.uint8)
+        fused = []
+        for ref in range(n):
+            ids = [ref] + list(sc.pairs[ref])
+            od, _, _ = O.fuse_view([cams[i] for i in ids], [dsc[i] for i in ids],
+                                   [ns[i] for i in ids], masks, ref, ids[1:], 3, 2.0, 0.01, cos_t,
+                                   precision=prec)
+            fused.append(od > 0)
+        return masks, np.stack(fused)
+
+    flips = {}
+    for prec in (32, 64):
+        m1, f1 = run(1.0, prec)
+        m2, f2 = run(1000.0, prec)
+        assert f1.sum() > 30000
+        flips[prec] = (int((m1 != m2).sum()), int((f1 != f2).sum()))
+    print(f"decisions changed by the unit change: FP32 {flips[32]}, FP64 {flips[64]}")
+    assert flips[64][0] == 0 and flips[64][1] <= 2
+    assert flips[64][0] + flips[64][1] < flips[32][0] + flips[32][1]

@@ -1,4 +1,6 @@
-"""GPU fusion vs the sequential FP32 twin: bit-exact masks, outputs, counts."""
+"""GPU fusion vs the sequential twin: bit-exact masks, outputs, counts (the
+reference's FP32 check, and the FP64 check of gmvs_fuse_view_fp64 against the
+twin's FP64 instantiation)."""
 import math
 
 import numpy as np
@@ -25,8 +27,9 @@ def noisy_maps(sc, rng, noise=0.01, outliers=0.05):
     return ds, ns
 
 
+@pytest.mark.parametrize("precision", [32, 64])
 @pytest.mark.parametrize("thresholds", [(2.0, 0.01, 0.174533, 3), (1.0, 0.01, math.radians(30), 2)])
-def test_fusion_bit_exact_against_twin(thresholds):
+def test_fusion_bit_exact_against_twin(thresholds, precision):
     from gmvs_amd import src as S
 
     reproj, rel, ang, gc = thresholds
@@ -50,11 +53,11 @@ def test_fusion_bit_exact_against_twin(thresholds):
         od = torch.empty(h, w, device=dev)
         on = torch.empty(h, w, 3, device=dev)
         S.fuse_view(ids, [cams_c[i] for i in ids], [dd[i] for i in ids], [nd[i] for i in ids],
-                    masks_g, gc, od, on, reproj, rel, ang)
+                    masks_g, gc, od, on, reproj, rel, ang, precision=precision)
         torch.cuda.synchronize()
         od_o, on_o, _ = O.fuse_view([ocams[i] for i in ids], [ds[i] for i in ids],
                                     [ns[i] for i in ids], masks_o, ref, ids[1:], gc, reproj, rel,
-                                    cos_t)
+                                    cos_t, precision=precision)
         gd, gn = od.cpu().numpy(), on.cpu().numpy()
         assert np.array_equal(gd.view(np.uint32), od_o.view(np.uint32)), f"depth of view {ref}"
         assert np.array_equal(gn.view(np.uint32), on_o.view(np.uint32)), f"normal of view {ref}"

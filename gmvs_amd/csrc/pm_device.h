@@ -553,8 +553,8 @@ constexpr int kBufWord3 = 0x00020000;
 //                FP32 path's results bit for bit with half its gather bytes;
 //   PREC_BF16 -- the cell in BF16, 8 bytes, and the bilinear sample s - c as
 //                two BF16 dot products with FP32 accumulation and one FP32
-//                FMA: dot((v00, v01-v00), (1, fx)) - c
-//                + fy * dot((v10-v00, d11), (1, fx)). fx carries BF16
+//                FMA: dot((v00, v01-v00), (1, fx))
+//                + [fy * dot((v10-v00, d11), (1, fx)) - c]. fx carries BF16
 //                rounding (8-bit mantissa); fy stays FP32.
 // The NCC moments are accumulated in FP32 about the centre intensity in
 // every mode.
@@ -607,9 +607,12 @@ struct Cell<PREC_BF16> {
         // both rows of the cell against one weight pair (1, fx): the row
         // values v00 + fx d01 and d10 + fx d11, then the FP32 row blend by fy
         const bf16x2 a = __builtin_convertvector((f32x2){1.0f, fx}, bf16x2);
-        const float top = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q0), a, -c, false);
+        // the products are selected as the accumulate-in-place form
+        // (v_dot2c): the lower row's blend fy * (d10 + fx d11) - c becomes
+        // the accumulator of the upper row's dot product, so only the first
+        // product needs its accumulator set (to zero)
         const float dyv = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q1), a, 0.0f, false);
-        return fmaf(fy, dyv, top);
+        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, q0), a, fmaf(fy, dyv, -c), false);
     }
 };
 

@@ -1191,6 +1191,7 @@ struct RefineSmem {
     float hp[kK2Pix][4];               // 1 / sum w, mean_r, var_r, centre intensity
     float4 tp[kRefine][kK2Pix];        // refinement hypotheses
     float4 tm[kRefine][kK2Pix];        // their m'' in the tile's frame (w unused)
+    float gv[kRefine][kK2Pix];         // geometric terms of a lane group's current unit
     int nunits;
 };
 
@@ -1284,6 +1285,17 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
         const Frame& fr = s_fr[v];
         const TapSrc src =
             tap_src_at<PR>(P.cells, (uint32_t)(P.nv * P.vcells), v * P.vcells, fr);
+        if (P.geom) {
+            // the unit's geometric terms, hypothesis k on lane k mod L of
+            // the group (each lane computing all five repeats the same
+            // arithmetic L times); the group's lanes are in one wave, whose
+            // LDS accesses complete in order
+#pragma unroll 1
+            for (int k = sub; k < kRefine; k += L) S.gv[k][g] = geom_cost(rc, vd, S.tp[k][sl], fpx, fpy);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
 #pragma unroll 1
         for (int k = 0; k < kRefine; ++k) {
             const float4 tm = S.tm[k][sl];
@@ -1291,7 +1303,7 @@ __device__ inline void pm_refine_body(const PMConst& P, int tile, int color,
             float H[9];
             homography(fr.A, fr.b, m, H);
             float x = ncc_eval_mlp<NT, L, PR>(hs, H, src, P.grid());
-            if (P.geom) x = x + 0.1f * geom_cost(rc, vd, S.tp[k][sl], fpx, fpy);
+            if (P.geom) x = x + 0.1f * S.gv[k][g];
             if (sub == 0) res[(k * kK2Pix + sl) * P.nv + v] = x;
         }
         if (sub == 0) {

@@ -131,7 +131,8 @@ def make_scene(width: int, height: int, num_views: int, num_planes: int, seed: i
                pitch_deg: Sequence[float] = (0.0,), depth_range: Tuple[float, float] = (2.0, 6.0),
                bump_fraction: float = 0.0, num_labels: int = 0, cap_deg: float = 60.0,
                feature_px: float = 6.0, device: str = "cpu",
-               cam_depth_range: Tuple[float, float] = (2.5, 35.0 / 6.0)) -> Scene:
+               cam_depth_range: Tuple[float, float] = (2.5, 35.0 / 6.0),
+               depth_band: Optional[Tuple[float, float]] = None) -> Scene:
     gen = np.random.default_rng(seed)
     if fx is None:
         fx = 300.0 * width / 320.0
@@ -142,27 +143,6 @@ def make_scene(width: int, height: int, num_views: int, num_planes: int, seed: i
 
     # Voronoi seeds in view 0 and planes through them
     seeds = np.stack([gen.uniform(0, width, num_planes), gen.uniform(0, height, num_planes)], 1)
-    planes = np.zeros((num_planes, 4))
-    cosmax = math.cos(math.radians(cap_deg))
-    C0 = c0.center
-    for i in range(num_planes):
-        # uniform on the cap around -z of camera 0 (facing the camera)
-        u = gen.uniform(cosmax, 1.0)
-        phi = gen.uniform(0, 2 * math.pi)
-        s = math.sqrt(1 - u * u)
-        n_cam = np.array([s * math.cos(phi), s * math.sin(phi), -u])
-        depth = gen.uniform(*depth_range)
-        ray = np.array([(seeds[i, 0] - c0.K[0, 2]) / c0.K[0, 0], (seeds[i, 1] - c0.K[1, 2]) / c0.K[1, 1], 1.0])
-        Xc = ray * depth
-        n_w = c0.R.T @ n_cam
-        Xw = c0.R.T @ Xc + C0
-        planes[i, :3] = n_w
-        planes[i, 3] = -n_w @ Xw
-    # background plane behind everything, facing view 0
-    zb = depth_range[1] * 1.6
-    bg = np.concatenate([c0.R.T @ np.array([0, 0, -1.0]), [0.0]])
-    bg[3] = -bg[:3] @ (c0.R.T @ np.array([0, 0, zb]) + C0)
-
     ys, xs = torch.meshgrid(torch.arange(height, device=dev, dtype=torch.float32),
                             torch.arange(width, device=dev, dtype=torch.float32), indexing="ij")
     sd = torch.tensor(seeds, dtype=torch.float32, device=dev)
@@ -173,6 +153,48 @@ def make_scene(width: int, height: int, num_views: int, num_planes: int, seed: i
         m = d2 < best
         owner0 = torch.where(m, torch.full_like(owner0, i), owner0)
         best = torch.minimum(best, d2)
+
+    planes = np.zeros((num_planes, 4))
+    cosmax = math.cos(math.radians(cap_deg))
+    C0 = c0.center
+    if depth_band is not None:
+        # camera-0 rays of every pixel, for the depth of a plane over its cell
+        rays0 = np.stack([(xs.cpu().numpy().astype(np.float64) - c0.K[0, 2]) / c0.K[0, 0],
+                          (ys.cpu().numpy().astype(np.float64) - c0.K[1, 2]) / c0.K[1, 1],
+                          np.ones((height, width))], -1)
+        own_np = owner0.cpu().numpy()
+    for i in range(num_planes):
+        ray = np.array([(seeds[i, 0] - c0.K[0, 2]) / c0.K[0, 0], (seeds[i, 1] - c0.K[1, 2]) / c0.K[1, 1], 1.0])
+        for _ in range(10000):
+            # uniform on the cap around -z of camera 0 (facing the camera);
+            # cap_deg = 90 is the whole facing hemisphere
+            u = gen.uniform(cosmax, 1.0)
+            phi = gen.uniform(0, 2 * math.pi)
+            s = math.sqrt(1 - u * u)
+            n_cam = np.array([s * math.cos(phi), s * math.sin(phi), -u])
+            depth = gen.uniform(*depth_range)
+            Xc = ray * depth
+            if depth_band is None:
+                break
+            # with a depth band, a plane is drawn again until its depth over
+            # its Voronoi cell stays inside the band (a plane close to
+            # grazing leaves any band, or the half-space in front of the
+            # camera, within its cell): the normals are uniform on the
+            # hemisphere given that condition
+            den = rays0[own_np == i] @ n_cam
+            dcell = (n_cam @ Xc) / np.where(den == 0.0, np.nan, den)
+            if np.all(np.isfinite(dcell)) and dcell.min() >= depth_band[0] and dcell.max() <= depth_band[1]:
+                break
+        else:
+            raise RuntimeError(f"plane {i}: no normal keeps its cell inside depth_band {depth_band}")
+        n_w = c0.R.T @ n_cam
+        Xw = c0.R.T @ Xc + C0
+        planes[i, :3] = n_w
+        planes[i, 3] = -n_w @ Xw
+    # background plane behind everything, facing view 0
+    zb = depth_range[1] * 1.6
+    bg = np.concatenate([c0.R.T @ np.array([0, 0, -1.0]), [0.0]])
+    bg[3] = -bg[:3] @ (c0.R.T @ np.array([0, 0, zb]) + C0)
 
     # Gaussian bumps: world centres on random view-0 surface points
     n_bumps = 0

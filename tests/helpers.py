@@ -14,8 +14,21 @@ SEED = 3407
 
 @functools.lru_cache(maxsize=8)
 def scene(width=320, height=240, views=5, planes=6, seed=SEED + 1, labels=0, bump=0.0):
+    # plane normals uniform on a 60-degree cap around the direction facing
+    # camera 0 (the suite's thresholds and known answers are set on it)
     return make_scene(width, height, views, planes, seed=seed, num_labels=labels,
                       bump_fraction=bump)
+
+
+@functools.lru_cache(maxsize=2)
+def hemisphere_scene(seed=SEED + 1):
+    """The CPU-runnable configuration as written: 6 planes whose normals are
+    uniform on the hemisphere facing camera 0, depths U[2, 6] at their seeds,
+    5 views at 320x240 (fx = fy = 300) on a 30-degree arc. A plane close to
+    grazing leaves any depth range within its Voronoi cell, so each plane is
+    drawn again until its depth over the cell stays inside the search range
+    [1.5, 7]: the normals are uniform on the hemisphere given that."""
+    return make_scene(320, 240, 5, 6, seed=seed, cap_deg=90.0, depth_band=(1.5, 7.0))
 
 
 def oracle_cams(sc, ids):

@@ -472,7 +472,8 @@ def main() -> int:
     ap.add_argument("--variants", default="plain,fp16,bf16",
                     help="other runs of the same workload: plain (FP32, one view per call), "
                          "fp16 / bf16 (batched, 16-bit samples), fused (batched FP32 with the "
-                         "view selection fused into the candidate kernel); comma list, empty "
+                         "view selection fused into the candidate kernel), devprior (batched "
+                         "FP32 with the planar priors built on the device); comma list, empty "
                          "for none")
     ap.add_argument("--fuse-select", action="store_true",
                     help="run the view selection as the candidate kernel's tile epilogue in the "
@@ -610,16 +611,19 @@ def main() -> int:
     specs = []
     for v in [v for v in args.variants.split(",") if v]:
         if v == "plain":
-            specs.append(("fp32_per_view", N.PREC_FP32, False, False))
+            specs.append(("fp32_per_view", N.PREC_FP32, False, False, False))
         elif v == "fused":
             # selection and adoption as the candidate kernel's tile epilogue
-            specs.append(("fp32_batched_fused_select", N.PREC_FP32, True, True))
+            specs.append(("fp32_batched_fused_select", N.PREC_FP32, True, True, False))
+        elif v == "devprior":
+            # planar priors built on the device in stream order
+            specs.append(("fp32_batched_device_prior", N.PREC_FP32, True, False, True))
         else:
-            specs.append((f"{v}_batched", N.PRECISIONS[v], True, False))
+            specs.append((f"{v}_batched", N.PRECISIONS[v], True, False, False))
     variants = {}
-    for name, prec, batched, fuse_sel in specs:
+    for name, prec, batched, fuse_sel, dev_prior in specs:
         vp = MultiViewPipeline(sc.images, cams, pairs, labels=labels, prior_threads=8,
-                               precision=prec, batched=batched)
+                               precision=prec, batched=batched, device_prior=dev_prior)
         N.set_select_fusion(fuse_sel)
         for i in range(args.warmup):
             step(vp)

@@ -139,6 +139,8 @@ EXPORTS = (
     "gmvs_prior_device_workspace_bytes",
     "gmvs_prior_build_device",
     "gmvs_delaunay",
+    "gmvs_delaunay_cells_device",
+    "gmvs_delaunay_cells_host",
     "gmvs_points_workspace_bytes",
     "gmvs_points_gather",
     "gmvs_format_workspace_bytes",
@@ -237,6 +239,9 @@ def _load() -> ctypes.CDLL:
     lib.gmvs_prior_build.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,
                                      ctypes.POINTER(CParams), vp, vp, ctypes.POINTER(i32)]
     lib.gmvs_delaunay.argtypes = [vp, i32, vp, i32, ctypes.POINTER(i32)]
+    lib.gmvs_delaunay_cells_device.argtypes = [vp, i32, i32, vp, i32, ctypes.POINTER(i32), vp, sz,
+                                               vp]
+    lib.gmvs_delaunay_cells_host.argtypes = [vp, i32, i32, vp, i32, ctypes.POINTER(i32)]
     u64 = ctypes.c_uint64
     lib.gmvs_points_workspace_bytes.argtypes = [i32, i32, i32]
     lib.gmvs_points_workspace_bytes.restype = sz

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "abi_common.h"
+#include "delaunay.h"
 
 namespace gmvs {
 namespace {
@@ -72,20 +73,14 @@ inline int orient(const Pt& a, const Pt& b, const Pt& c) {
     return limit_sign(r);
 }
 
-// true when the input point d lies strictly inside the circumcircle of the
-// positively oriented triangle (a, b, c).
-inline bool in_circle(const Pt& a, const Pt& b, const Pt& c, const Pt& d) {
-    if (a.real() && b.real() && c.real()) {
-        const i128 adx = a.x - d.x, ady = a.y - d.y;
-        const i128 bdx = b.x - d.x, bdy = b.y - d.y;
-        const i128 cdx = c.x - d.x, cdy = c.y - d.y;
-        const i128 ad = adx * adx + ady * ady;
-        const i128 bd = bdx * bdx + bdy * bdy;
-        const i128 cd = cdx * cdx + cdy * cdy;
-        const i128 det = ad * (bdx * cdy - bdy * cdx) - bd * (adx * cdy - ady * cdx) +
-                         cd * (adx * bdy - ady * bdx);
-        return det > 0;
-    }
+// true when the input point d lies inside the circumcircle of the
+// positively oriented triangle (a, b, c); ia..id are the point indices,
+// which order the symbolic perturbation of cocircular input points
+// (dt::in_circle, shared with the device construction).
+inline bool in_circle(const Pt& a, int ia, const Pt& b, int ib, const Pt& c, int ic, const Pt& d,
+                      int id) {
+    if (a.real() && b.real() && c.real())
+        return dt::in_circle(a.x, a.y, ia, b.x, b.y, ib, c.x, c.y, ic, d.x, d.y, id);
     const Lin X[3] = {sub(a, d), sub(b, d), sub(c, d)};
     const Lin Y[3] = {suby(a, d), suby(b, d), suby(c, d)};
     i128 lift[3][3], cross[3][3];  // lift[k] = |k - d|^2; cross[k] = the 2x2 minor without k
@@ -160,16 +155,23 @@ public:
         for (int i : order) insert(i);
     }
 
-    // Triangles whose three vertices are input points.
+    // Triangles whose three vertices are input points, counter-clockwise
+    // from their lowest index, listed by that index and then by the next
+    // vertex's (the device construction's order).
     std::vector<int> triangles() const {
-        std::vector<int> out;
+        std::vector<std::array<int, 3>> t3;
         for (const Tri& t : tris_) {
             if (!t.alive) continue;
             if (t.v[0] >= n_ || t.v[1] >= n_ || t.v[2] >= n_) continue;
-            out.push_back(t.v[0]);
-            out.push_back(t.v[1]);
-            out.push_back(t.v[2]);
+            int k = 0;
+            if (t.v[1] < t.v[k]) k = 1;
+            if (t.v[2] < t.v[k]) k = 2;
+            t3.push_back({t.v[k], t.v[(k + 1) % 3], t.v[(k + 2) % 3]});
         }
+        std::sort(t3.begin(), t3.end());
+        std::vector<int> out;
+        out.reserve(3 * t3.size());
+        for (const auto& t : t3) out.insert(out.end(), t.begin(), t.end());
         return out;
     }
 
@@ -256,7 +258,7 @@ private:
                 const int n = tris_[t].nb[e];
                 if (n < 0 || mark_[n] == stamp_) continue;
                 const Tri& N = tris_[n];
-                if (in_circle(p_[N.v[0]], p_[N.v[1]], p_[N.v[2]], q)) {
+                if (in_circle(p_[N.v[0]], N.v[0], p_[N.v[1]], N.v[1], p_[N.v[2]], N.v[2], q, pi)) {
                     mark_[n] = stamp_;
                     bad.push_back(n);
                     stack.push_back(n);

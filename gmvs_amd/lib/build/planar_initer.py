@@ -55,12 +55,15 @@ def build_prior(ref_cam: N.CCamera, depth: np.ndarray, cost: np.ndarray,
 
 
 def build_prior_device(ref_cam: N.CCamera, depth, cost, labels, depth_min: float,
-                       depth_max: float, superpixel_filter: bool = False):
+                       depth_max: float, superpixel_filter: bool = False, count: bool = True):
     """Device planar-prior construction (gmvs_prior_build_device; equal to
     build_prior bit for bit). depth, cost: [H,W] float32 device tensors;
     labels: [H,W] int32 device tensor or None. Returns (prior_planes [H,W,4]
-    f32, plane_masks [H,W] int32, #triangles), device tensors; synchronises
-    the current stream (the triangulation runs on the host)."""
+    f32, plane_masks [H,W] int32, #triangles), device tensors. Every step,
+    the Delaunay triangulation included, runs on the current stream; with
+    count=True the call waits for it to report the number of retained
+    triangles, with count=False it only enqueues the work (#triangles is
+    None)."""
     import torch
 
     h, w = ref_cam.height, ref_cam.width
@@ -82,8 +85,8 @@ def build_prior_device(ref_cam: N.CCamera, depth, cost, labels, depth_min: float
     N.check("gmvs_prior_build_device", N.lib.gmvs_prior_build_device(
         ctypes.byref(ref_cam), depth.data_ptr(), cost.data_ptr(),
         lab.data_ptr() if lab is not None else None, ctypes.byref(cp), planes.data_ptr(),
-        masks.data_ptr(), ctypes.byref(ntri), ws.data_ptr(), nbytes, stream))
-    return planes, masks, int(ntri.value)
+        masks.data_ptr(), ctypes.byref(ntri) if count else None, ws.data_ptr(), nbytes, stream))
+    return planes, masks, (int(ntri.value) if count else None)
 
 
 def delaunay(xy: np.ndarray) -> np.ndarray:
@@ -99,6 +102,35 @@ def delaunay(xy: np.ndarray) -> np.ndarray:
     N.check("gmvs_delaunay", N.lib.gmvs_delaunay(pts.ctypes.data, len(pts), tri.ctypes.data,
                                                  ntri.value, ctypes.byref(ntri)))
     return tri[:ntri.value]
+
+
+def delaunay_cells(cell_xy: np.ndarray, device=None) -> np.ndarray:
+    """The triangulation of points on a grid of 5x5 pixel cells, at most one
+    per cell (gmvs_delaunay_cells_device, or _host with device=None: the
+    same construction on the host). cell_xy: [ncx, ncy, 2] int32, (x, y)
+    inside the cell or (-1, -1). Returns [T,3] int32 indices of the points
+    numbered in cell order (column-major), listed as delaunay() lists the
+    same points."""
+    c = np.ascontiguousarray(cell_xy, np.int32)
+    ncx, ncy = int(c.shape[0]), int(c.shape[1])
+    ntri = ctypes.c_int32(0)
+    max_tri = 2 * ncx * ncy + 8
+    if device is None:
+        tri = np.zeros((max_tri, 3), np.int32)
+        N.check("gmvs_delaunay_cells_host", N.lib.gmvs_delaunay_cells_host(
+            c.ctypes.data, ncx, ncy, tri.ctypes.data, max_tri, ctypes.byref(ntri)))
+        return tri[:ntri.value]
+    import torch
+
+    dc = torch.from_numpy(c).to(device)
+    out = torch.zeros(max_tri, 3, dtype=torch.int32, device=device)
+    nbytes = N.lib.gmvs_prior_device_workspace_bytes(5 * ncy, 5 * ncx)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    N.check("gmvs_delaunay_cells_device", N.lib.gmvs_delaunay_cells_device(
+        dc.data_ptr(), ncx, ncy, out.data_ptr(), max_tri, ctypes.byref(ntri), ws.data_ptr(), nbytes,
+        stream))
+    return out[:ntri.value].cpu().numpy()
 
 
 class PlanarInit:
@@ -212,8 +244,7 @@ class PlanarInit:
         if self._superpixel is not None:
             lab = torch.from_numpy(np.ascontiguousarray(self._superpixel, np.int32)).to(dev)
         # on the device (gmvs_prior_build_device, equal to the host build bit
-        # for bit): the cost map stays resident, only the support points and
-        # triangles cross to the host for the Delaunay step
+        # for bit): the cost map stays resident, the Delaunay step included
         planes, masks, ntri = build_prior_device(cam._c, d, self._costs, lab,
                                                  self.params.depth_min, self.params.depth_max,
                                                  bool(self.params.superpixel_filter)

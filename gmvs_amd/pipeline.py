@@ -8,9 +8,12 @@ gmvs/scripts/launch.py:122-177 without the per-stage file round trips.
     fuse()         sequential multi-view fusion (consistency.py:141-211)
 
 All views stay resident in HBM. GPU work is issued on the current torch
-stream; the host prior construction of a view (support points, Delaunay,
-plane fit) runs on a thread pool as soon as that view's first pass has
-finished on the device, overlapping GPU work of other views. Launch
+stream. The planar prior of a view (support points, Delaunay, plane fit) is
+built either on the host, on a thread pool as soon as that view's first pass
+has finished on the device, overlapping GPU work of other views; or, with
+device_prior=True, on the device (gmvs_prior_build_device: no copies to the
+host and no host synchronisation; the same priors bit for bit), on a second
+stream that overlaps it with the other group's PatchMatch kernels. Launch
 submission is serialised on the calling thread, so the workspaces are reused
 in stream order.
 
@@ -33,7 +36,7 @@ import torch
 
 from . import _native as N
 from . import src as S
-from .lib.build.planar_initer import build_prior
+from .lib.build.planar_initer import build_prior, build_prior_device
 
 
 @dataclass
@@ -48,7 +51,7 @@ class MultiViewPipeline:
                  pairs: Sequence[Sequence[int]], labels: Optional[Sequence[np.ndarray]] = None,
                  params: Optional[S.PatchMatchParams] = None, prior_threads: int = 8,
                  precision: int = N.PREC_FP32, batched: bool = True, groups: int = 2,
-                 tile_cols: int = 0):
+                 tile_cols: int = 0, device_prior: bool = False):
         self.dev = S._device()
         self.images = images.to(self.dev, torch.float32).contiguous()
         self.n, self.h, self.w = (int(x) for x in self.images.shape)
@@ -57,6 +60,7 @@ class MultiViewPipeline:
         self.labels = labels
         self.params = params if params is not None else S.PatchMatchParams()
         self.prior_threads = prior_threads
+        self.device_prior = bool(device_prior)
         self.precision = S.resolve_precision(precision, self.images)
         self.batched = batched
         self.tile_cols = tile_cols
@@ -105,6 +109,9 @@ class MultiViewPipeline:
         groups = [list(g) for g in np.array_split(np.arange(n), self.groups) if len(g)]
         if not hasattr(self, "_ws_groups") or len(self._ws_groups) != len(groups):
             self._ws_groups = [S._Workspace() for _ in groups]
+        if planar_prior and self.device_prior:
+            return self._first_stage_device_prior(iterations, superpixel_filter, groups, planes,
+                                                  costs, out_d, out_n, out_c)
         done = []
         for gi, g in enumerate(groups):
             S.run_pm_batch([self._problem(r, iterations, False, init_mode=N.INIT_RANDOM, geom=False,
@@ -146,6 +153,59 @@ class MultiViewPipeline:
                                               out_normal=out_n[r], out_cost=out_c[r])
                                 for r in g], self._ws_groups[gi], dev)
             torch.cuda.current_stream().synchronize()  # pinned buffers in `keep` outlive the copies
+        return StageOutput(out_d, out_n, out_c)
+
+    def _first_stage_device_prior(self, iterations: int, superpixel_filter: bool, groups,
+                                  planes, costs, out_d, out_n, out_c) -> StageOutput:
+        """The batched first stage with the planar priors built on the device:
+        each group's priors on a second stream (default priority) as soon as
+        its plain pass is done, overlapping the next group's plain pass and
+        the previous group's prior pass, which waits for them by event."""
+        n, h, w, dev = self.n, self.h, self.w, self.dev
+        prior_planes = torch.empty(n, h, w, 4, device=dev)
+        plane_masks = torch.empty(n, h, w, dtype=torch.int32, device=dev)
+        labels = None
+        if self.labels is not None and superpixel_filter:
+            labels = [torch.from_numpy(np.ascontiguousarray(self.labels[r], np.int32)).to(dev)
+                      for r in range(n)]
+        self.num_triangles = [None] * n
+        main = torch.cuda.current_stream()
+        if getattr(self, "_side", None) is None:
+            self._side = torch.cuda.Stream(device=dev)
+        side = self._side
+        built = []
+        for gi, g in enumerate(groups):
+            S.run_pm_batch([self._problem(r, iterations, False, init_mode=N.INIT_RANDOM, geom=False,
+                                          planes=planes[r], costs=costs[r], out_depth=out_d[r],
+                                          out_normal=out_n[r], out_cost=out_c[r])
+                            for r in g], self._ws_groups[gi], dev)
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                for r in g:
+                    cp = self._cparams(r, iterations, True)
+                    pl, mk, _ = build_prior_device(self.cameras[r]._c, out_d[r], out_c[r],
+                                                   None if labels is None else labels[r],
+                                                   cp.depth_min, cp.depth_max, labels is not None,
+                                                   count=False)
+                    prior_planes[r].copy_(pl)
+                    plane_masks[r].copy_(mk)
+                ev = torch.cuda.Event()
+                ev.record(side)
+                built.append(ev)
+        for gi, g in enumerate(groups):
+            main.wait_event(built[gi])
+            S.run_pm_batch([self._problem(r, iterations, True, init_mode=N.INIT_PRIOR, geom=False,
+                                          planes=planes[r], costs=costs[r],
+                                          prior_planes=prior_planes[r], plane_masks=plane_masks[r],
+                                          out_depth=out_d[r], out_normal=out_n[r], out_cost=out_c[r])
+                            for r in g], self._ws_groups[gi], dev)
+        # the buffers the side stream used are reused by later work on the main stream
+        main.wait_stream(side)
+        for t in (out_d, out_c, prior_planes, plane_masks):
+            t.record_stream(side)
+        if labels is not None:
+            for t in labels:
+                t.record_stream(side)
         return StageOutput(out_d, out_n, out_c)
 
     def _geom_pass_batched(self, prev: StageOutput, iterations: int) -> StageOutput:

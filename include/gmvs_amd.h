@@ -29,7 +29,8 @@ enum {
     GMVS_ERR_ARG = 1,      /* invalid argument (shape, count, NULL pointer) */
     GMVS_ERR_HIP = 2,      /* a HIP runtime call failed */
     GMVS_ERR_NOSPACE = 3,  /* workspace too small */
-    GMVS_ERR_UNSUPPORTED = 4
+    GMVS_ERR_UNSUPPORTED = 4,
+    GMVS_ERR_INTERNAL = 5  /* a device-side consistency check failed */
 };
 
 /* mvs::Camera (gmvs/include/data_structure.hpp:45-53): K, world->camera R, t
@@ -282,11 +283,14 @@ int gmvs_prior_build(const gmvs_camera* ref_camera, const float* depth, const fl
                      uint32_t* plane_masks, int32_t* out_num_triangles);
 
 /* gmvs_prior_build on the device, with the same results bit for bit:
- * support points, the superpixel filter, plane fit, rasterisation and the
- * depth-range check run as kernels; the Delaunay triangulation between them
- * runs on the host (the kept support points are copied back, the triangles
- * up), so the call synchronises the stream. Replaces the same reference
- * steps as gmvs_prior_build (gmvs/lib/src/planar_initer.cu:195-420,686-745).
+ * support points, their Delaunay triangulation (one star per point, the
+ * host construction's predicates and order), the superpixel filter, plane
+ * fit, rasterisation and the depth-range check, all as kernels on the
+ * stream. With out_num_triangles NULL the call only enqueues work (no
+ * synchronisation); with it, the call waits for the stream, reports the
+ * number of retained triangles and returns GMVS_ERR_INTERNAL if the device
+ * triangulation failed its own checks. Replaces the same reference steps as
+ * gmvs_prior_build (gmvs/lib/src/planar_initer.cu:195-420,686-745).
  * depth, cost: device [H*W] f32; labels: device [H*W] int32 or NULL;
  * prior_planes: device [H*W*4]; plane_masks: device [H*W];
  * workspace: device, gmvs_prior_device_workspace_bytes(H, W) bytes. */
@@ -303,6 +307,27 @@ int gmvs_prior_build_device(const gmvs_camera* ref_camera, const float* depth, c
  * indices (counter-clockwise) or NULL; *num_tri receives the count. */
 int gmvs_delaunay(const int32_t* xy, int32_t n, int32_t* tri_out, int32_t max_tri,
                   int32_t* num_tri);
+
+/* The device triangulation of gmvs_prior_build_device on its own, for tests
+ * and callers with support points of their own; replaces the same Subdiv2D
+ * calls (gmvs/lib/src/planar_initer.cu:354-361). Points on a grid of 5x5
+ * pixel cells, at most one per cell: cell_xy is device [ncx*ncy*2] int32,
+ * cell (cx, cy) at index cx*ncy + cy holding (x, y) inside the cell
+ * (x / 5 == cx, y / 5 == cy) or (-1, -1); points are numbered in cell
+ * order. tri_out: device [max_tri*3], the triangles as gmvs_delaunay lists
+ * them for the same numbered points (the same triangles in the same
+ * order); *num_tri (host) the count. Synchronises the stream.
+ * workspace: device, gmvs_prior_device_workspace_bytes(5*ncy, 5*ncx) bytes.
+ * Returns GMVS_ERR_ARG for a point outside its cell, GMVS_ERR_NOSPACE when
+ * the triangles exceed max_tri. */
+int gmvs_delaunay_cells_device(const int32_t* cell_xy, int32_t ncx, int32_t ncy, int32_t* tri_out,
+                               int32_t max_tri, int32_t* num_tri, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
+/* The same star construction run on the host (the device kernel's code),
+ * for tests on machines without a device. cell_xy, tri_out: host. */
+int gmvs_delaunay_cells_host(const int32_t* cell_xy, int32_t ncx, int32_t ncy, int32_t* tri_out,
+                             int32_t max_tri, int32_t* num_tri);
 
 /* Point gather after fusion -- the "Gather mvs points" loop of
  * gmvs/scripts/mvs_fusion_segmentaion_cpu.py:158-205. For each view, the

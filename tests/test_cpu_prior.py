@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from gmvs_amd import _native as N
-from gmvs_amd.lib.build.planar_initer import build_prior, delaunay
+from gmvs_amd.lib.build.planar_initer import build_prior, delaunay, delaunay_cells
 
 
 def _random_points(n, seed, w=1600, h=1200):
@@ -95,6 +95,72 @@ def test_delaunay_degenerate_inputs():
     assert len(tri) == 2 * len(grid) - 2 - _hull_count(grid)
     with pytest.raises(ValueError):
         delaunay(np.array([[-1, 3], [4, 5], [6, 0]]))
+
+
+def cell_points(ncx, ncy, seed, keep=0.8, mode="random", holes=()):
+    """Points on a grid of 5x5 cells, at most one per cell, as the support
+    points are: [ncx, ncy, 2] with (-1, -1) for an empty cell. mode "random":
+    any pixel of the cell; "corner": the cell's corner (a square lattice,
+    every cell's four points cocircular); "two": one of the cell's two
+    diagonal corners. holes: (cx, cy, r) discs of empty cells."""
+    rng = np.random.default_rng(seed)
+    cx, cy = np.meshgrid(np.arange(ncx), np.arange(ncy), indexing="ij")
+    if mode == "random":
+        ox, oy = rng.integers(0, 5, (ncx, ncy)), rng.integers(0, 5, (ncx, ncy))
+    elif mode == "corner":
+        ox = oy = np.zeros((ncx, ncy), np.int64)
+    else:
+        ox = oy = 4 * rng.integers(0, 2, (ncx, ncy))
+    on = rng.random((ncx, ncy)) < keep
+    for hx, hy, r in holes:
+        on &= (cx - hx) ** 2 + (cy - hy) ** 2 >= r * r
+    c = np.full((ncx, ncy, 2), -1, np.int32)
+    c[..., 0] = np.where(on, 5 * cx + ox, -1)
+    c[..., 1] = np.where(on, 5 * cy + oy, -1)
+    return c
+
+
+@pytest.mark.parametrize("ncx,ncy,seed,keep,mode,holes", [
+    (1, 1, 0, 1.0, "random", ()),
+    (1, 9, 1, 1.0, "random", ()),
+    (2, 2, 2, 1.0, "corner", ()),
+    (12, 9, 3, 1.0, "corner", ()),
+    (12, 9, 4, 0.6, "corner", ()),
+    (20, 15, 5, 0.9, "two", ()),
+    (20, 15, 6, 0.15, "random", ()),
+    (40, 30, 7, 0.8, "random", ((20, 15, 8),)),
+    (40, 30, 8, 0.5, "two", ((5, 5, 6), (30, 25, 9))),
+    (320, 240, 9, 0.7, "random", ((100, 80, 40), (250, 200, 25), (30, 200, 60))),
+])
+def test_star_construction_equals_incremental(ncx, ncy, seed, keep, mode, holes):
+    """The device construction's code (each point's star, run on the host)
+    against the incremental construction: both triangulate the perturbed
+    points, so the triangles and their order agree exactly, on lattices where
+    most quadruples are cocircular, on sparse sets and around large empty
+    regions (circumcircles far larger than the search square)."""
+    c = cell_points(ncx, ncy, seed, keep, mode, holes)
+    xy = c.reshape(-1, 2)
+    pts = xy[xy[:, 0] >= 0].astype(np.int64)
+    star = delaunay_cells(c)
+    inc = delaunay(pts) if len(pts) else np.zeros((0, 3), np.int32)
+    assert star.shape == inc.shape and np.array_equal(star, inc)
+    if len(star):
+        # listed by the lowest-index vertex, then by the next vertex
+        assert np.all(star[:, 0] < star[:, 1]) and np.all(star[:, 0] < star[:, 2])
+        key = star[:, 0].astype(np.int64) * len(pts) + star[:, 1]
+        assert np.all(np.diff(key) > 0)
+    if 3 <= len(pts) <= 4000:
+        assert _incircle_violations(pts, star) == 0
+        hull = _hull_count(pts)
+        if len(star):
+            assert len(star) == 2 * len(pts) - 2 - hull
+
+
+def test_star_construction_rejects_points_outside_their_cells():
+    c = cell_points(4, 4, 0)
+    c[1, 2] = (0, 0)  # belongs to cell (0, 0)
+    with pytest.raises(ValueError):
+        delaunay_cells(c)
 
 
 def _camera(h, w, f=200.0):

@@ -70,9 +70,12 @@ def test_prior_pass_divergences_are_fp32_ties():
                                                        ("refine", 8), ("ncc_jump", 16))}
     print(f"pixel updates {n}; divergent {len(ids)} ({len(ids) / n:.2%}); {counts}; "
           f"masked {int(masked.sum())}")
-    # the drawn views never differ: the draws and the view-selection PDF are
-    # far from their decision boundaries
-    assert counts["draw"] == 0 and counts["ncc_jump"] <= 3
+    # drawn views differ only where a draw lies within FP32 rounding of a CDF
+    # step or a table cost within FP32 rounding of tau or 1.2 (margin column
+    # 0), which happens a handful of times in 230,400 updates
+    assert counts["draw"] <= 3 and counts["ncc_jump"] <= 3
+    if counts["draw"]:
+        assert mg[what == 1, 0].max() < 1e-5, mg[what == 1, 0]
     # divergences are a property of the prior's masked branch
     assert (masked | (what == 8)).mean() > 0.995
     ulps = []

@@ -121,6 +121,28 @@ def test_pipeline_batched_equals_per_view():
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_pipeline_device_prior_equals_host_prior():
+    """The first stage with the planar priors built on the device in stream
+    order (no host copies or synchronisation) returns the host-prior
+    pipeline's bits."""
+    N, S = _native()
+    from gmvs_amd.pipeline import MultiViewPipeline, cameras_from_synthetic
+
+    sc = make_scene(240, 180, 5, 5, seed=3407 + 13)
+    cams = cameras_from_synthetic(sc.cameras)
+    pairs = [p[:4] for p in sc.pairs]
+    labels = [voronoi_labels(240, 180, 40, seed=3407 + 40 + i) for i in range(5)]
+    res = []
+    for device_prior in (True, False):
+        pipe = MultiViewPipeline(sc.images, cams, pairs, labels=labels, prior_threads=4,
+                                 device_prior=device_prior)
+        s1 = pipe.first_stage(iterations=2, planar_prior=True, superpixel_filter=True)
+        torch.cuda.synchronize()
+        res.append([t.cpu().numpy() for t in (s1.depth, s1.normal, s1.cost)])
+    for a, b in zip(*res):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_pipeline_auto_precision_equals_fp32():
     """precision="auto" picks FP16 cells for 8-bit imagery, which hold it
     exactly: the whole pipeline (prior, geometric pass) returns the FP32

@@ -103,6 +103,37 @@ def test_acmp_prior_pass_matches_fp64():
     assert agree >= agree32 - 0.01
 
 
+def test_acmp_device_prior_equals_host():
+    """The device planar-prior construction (Delaunay included) on the
+    support points of a real pass: view 0 of the 10-view workload after the
+    GPU plain pass (8 iterations), tens of thousands of support points with
+    the gaps a pass leaves (occlusions, slanted planes): equal to the host
+    construction bit for bit, and its time."""
+    import time
+
+    from gmvs_amd.lib.build import planar_initer as PI
+
+    sc, labels, ids, cams, imgs, kw = _acmp()
+    g = _pm_gpu(sc, ids, dict(kw, max_iterations=8))
+    cam = _ccams(sc, [0])[0]
+    hp, hm, hn = PI.build_prior(cam, g["depth"], g["cost"], labels[0], kw["depth_min"],
+                                kw["depth_max"], superpixel_filter=True)
+    dev = torch.device("cuda")
+    d, c = torch.from_numpy(g["depth"]).to(dev), torch.from_numpy(g["cost"]).to(dev)
+    lab = torch.from_numpy(np.ascontiguousarray(labels[0], np.int32)).to(dev)
+    args = (cam, d, c, lab, kw["depth_min"], kw["depth_max"])
+    PI.build_prior_device(*args, superpixel_filter=True)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gp, gm, gn = PI.build_prior_device(*args, superpixel_filter=True)
+    ms = (time.perf_counter() - t0) * 1e3
+    print(f"1600x1200 prior from a GPU plain pass: {hn} retained triangles (host), {gn} (device); "
+          f"device build {ms:.1f} ms")
+    assert gn == hn and hn > 1000
+    assert np.array_equal(gm.cpu().numpy(), hm)
+    assert np.array_equal(gp.cpu().numpy().view(np.uint32), hp.view(np.uint32))
+
+
 def test_acmp_geom_pass_matches_fp64():
     sc, labels, ids, cams, imgs, kw = _acmp()
     rng = np.random.default_rng(23)

@@ -111,10 +111,20 @@ __host__ __device__ inline int pencil_min(const Grid& g, int p, int q, int r, in
                : r;
 }
 
+// The group's lowest candidate, on every lane.
+template <int L>
+__host__ __device__ inline int group_pencil_min(const Grid& g, int p, int q, int r, Lanes<L> ln) {
+    (void)ln;
+    if constexpr (L > 1) {
+        for (int off = L / 2; off > 0; off >>= 1) r = pencil_min(g, p, q, r, __shfl_xor(r, off, L));
+    }
+    return r;
+}
+
 // Grows r over the points of the cells [x0, x1] x [y0, y1]: the point left
 // of p -> q whose circle through p and q is smallest on the left (r = -1:
 // none yet). Every lane of the group returns the same r.
-template <int L>
+template <int L, bool REDUCE = true>
 __host__ __device__ inline int left_min(const Grid& g, int p, int q, int x0, int x1, int y0, int y1,
                                         int r, Lanes<L> ln) {
     const int64_t pxv = g.px[p], pyv = g.py[p], qxv = g.px[q], qyv = g.py[q];
@@ -132,9 +142,7 @@ __host__ __device__ inline int left_min(const Grid& g, int p, int q, int x0, int
         for (int cx = x0; cx <= x1; ++cx)
             for (int cy = y0 + ln.lane; cy <= y1; cy += L) visit(cx, cy);
     }
-    if constexpr (L > 1) {
-        for (int off = L / 2; off > 0; off >>= 1) r = pencil_min(g, p, q, r, __shfl_xor(r, off, L));
-    }
+    if constexpr (L > 1 && REDUCE) r = group_pencil_min(g, p, q, r, ln);
     return r;
 }
 
@@ -182,9 +190,9 @@ __host__ __device__ inline int left_min_halfplane(const Grid& g, int p, int q, i
             }
             if (x0 > x1) continue;
         }
-        r = left_min(g, p, q, x0, x1, cy, cy, r, ln);
+        r = left_min<L, false>(g, p, q, x0, x1, cy, cy, r, ln);  // lane-local
     }
-    return r;
+    return group_pencil_min(g, p, q, r, ln);
 }
 
 // The third vertex of the Delaunay triangle left of the Delaunay edge

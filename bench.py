@@ -719,6 +719,11 @@ def main() -> int:
                    "height": h, "width": w, "sources": 9, "iterations": 8, "geom_iterations": 2,
                    "superpixels": 2000, "planes": 24, "pixel_hypotheses_per_step": ph_step,
                    "fused_points": fused},
+        # the NCCs the kernels evaluated (refinement costs only the selected
+        # views, so fewer than pixel-hypotheses x sources), per step and per
+        # second of the timed region
+        "ncc": {"per_step": int(total_ncc // args.steps),
+                "g_per_s": round(total_ncc / elapsed / 1e9, 2)},
         "fusion": fusion,
         "variants": variants,
         "roofline": roof,

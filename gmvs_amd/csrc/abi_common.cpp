@@ -30,7 +30,7 @@ namespace {
 const char* const kKernelNames[K_NUM] = {
     "pm_setup_view", "pm_pack_quads", "pm_init",      "pm_candidates",  "pm_select",  "pm_refine",
     "pm_depth_normal", "pm_filter",   "pm_export",    "pm_cost_batch",  "fuse_fill",
-    "fuse_check",     "fuse_scatter", "fuse_write_view", "points_gather", "format_rows",
+    "fuse_check",     "fuse_scatter", "fuse_write_views", "points_gather", "format_rows",
     "sliding_window_normal", "segment_sort", "segment_stats"};
 
 struct Pending {

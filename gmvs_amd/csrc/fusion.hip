@@ -19,6 +19,7 @@
 // with the oracle's FP64 instantiation of the same loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -87,8 +88,16 @@ __device__ inline bool fz_round_in(T x, int n, int* out) {
     return true;
 }
 
-__global__ void fuse_write_view(FuseView v, FuseView* __restrict__ dst) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
+// The view table goes to the device as kernel arguments, up to kViewChunk
+// views per launch (2 KB of arguments), so a call needs no host copy.
+constexpr int kViewChunk = 16;
+struct FuseViewChunk {
+    FuseView v[kViewChunk];
+};
+
+__global__ void fuse_write_views(FuseViewChunk c, int n, FuseView* __restrict__ dst) {
+    const int i = threadIdx.x;
+    if (blockIdx.x == 0 && i < n) dst[i] = c.v[i];
 }
 
 __global__ void fuse_fill(int hw, float* __restrict__ depth, float* __restrict__ normal,
@@ -278,13 +287,19 @@ int fuse_view_impl(bool fp64, int32_t num_images, const gmvs_camera* cameras,
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     char* ws = reinterpret_cast<char*>(workspace);
     FuseView* vtab = reinterpret_cast<FuseView*>(ws + L.views);
-    for (int i = 0; i < num_images; ++i) {
-        FuseView fv;
-        fv.cam = cameras[i];
-        fv.depth = depths[i];
-        fv.normal = normals[i];
-        fv.global_id = i ? src_ids[i - 1] : ref_id;
-        GMVS_LAUNCH(K_FUSE_VIEWDESC, st, fuse_write_view, dim3(1), dim3(64), 0, st, fv, vtab + i);
+    for (int i0 = 0; i0 < num_images; i0 += kViewChunk) {
+        FuseViewChunk chunk{};
+        const int n = std::min(kViewChunk, num_images - i0);
+        for (int k = 0; k < n; ++k) {
+            const int i = i0 + k;
+            FuseView& fv = chunk.v[k];
+            fv.cam = cameras[i];
+            fv.depth = depths[i];
+            fv.normal = normals[i];
+            fv.global_id = i ? src_ids[i - 1] : ref_id;
+        }
+        GMVS_LAUNCH(K_FUSE_VIEWDESC, st, fuse_write_views, dim3(1), dim3(kViewChunk), 0, st, chunk,
+                    n, vtab + i0);
         GMVS_LAUNCH_CHECK();
     }
     const int hw = h * w;

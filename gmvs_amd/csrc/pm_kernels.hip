@@ -107,19 +107,23 @@ __device__ inline void count_work(unsigned long long* ctr, uint32_t ncc, uint32_
     }
 }
 
-// The local frames (see Frame) of one tile of reference pixels for every
-// view of the problem, one thread per view: reference positions relative to
-// the tile origin (ox, oy), source positions relative to the image of the
-// tile's centre under the fronto-parallel plane at the middle inverse depth.
-// The block synchronises before the frames are read.
+// The local frame (see Frame) of one tile of reference pixels for view v:
+// reference positions relative to the tile origin (ox, oy), source positions
+// relative to the image of the tile's centre under the fronto-parallel plane
+// at the middle inverse depth. tile_frames writes the frames of every view of
+// the problem, one thread per view; the block synchronises before the frames
+// are read.
+__device__ inline void tile_frame(const PMConst& P, int v, int ox, int oy, int cols, int rows,
+                                  Frame* dst) {
+    double am[3];
+    fronto_m(0.5 * (1.0 / (double)P.dmin + 1.0 / (double)P.dmax), am);
+    *dst = make_frame(P.views64[v], ox, oy, ox + 0.5 * cols, oy + 0.5 * rows, am, P.w, P.h, P.wp);
+}
+
 __device__ inline void tile_frames(const PMConst& P, int ox, int oy, int cols, int rows,
                                    Frame* s_fr) {
-    if ((int)threadIdx.x < P.nv) {
-        double am[3];
-        fronto_m(0.5 * (1.0 / (double)P.dmin + 1.0 / (double)P.dmax), am);
-        s_fr[threadIdx.x] = make_frame(P.views64[threadIdx.x], ox, oy, ox + 0.5 * cols,
-                                       oy + 0.5 * rows, am, P.w, P.h, P.wp);
-    }
+    if ((int)threadIdx.x < P.nv)
+        tile_frame(P, (int)threadIdx.x, ox, oy, cols, rows, &s_fr[threadIdx.x]);
 }
 
 // ------------------------------------------------------------- batching ---
@@ -157,6 +161,7 @@ enum { WT_FULL = 0,  // 8x8-pixel tiles over the image (pm_init)
        WT_HALF,      // 8x8 tiles over one colour's active pixels (candidates, refine)
        WT_ACTIVE,    // 256-pixel chunks of active pixels (select, filter)
        WT_PIXELS,    // 256-pixel chunks of the image (depth/normal, export)
+       WT_HALF_PAIRS,  // pairs of horizontally adjacent WT_HALF tiles (candidates)
        WT_NUM };
 
 __device__ inline int problem_of(const WorkTable& t, int item) {
@@ -475,15 +480,16 @@ __device__ inline Active active_slot(const PMConst& P, int tile, int color, int 
     return a;
 }
 
+// Pixel of thread lt (of the tile's 64 * L threads) in an NCC kernel's tile.
 template <int L>
-__device__ inline Active active_pixel(const PMConst& P, int tile, int color) {
+__device__ inline Active active_pixel(const PMConst& P, int tile, int color, int lt) {
     const int TX = P.tile.tx(), TY = P.tile.ty();
     const int tx = (P.wh + TX - 1) / TX;
-    const int slot = threadIdx.x / L;
+    const int slot = lt / L;
     const int ax = (tile % tx) * TX + (slot & (TX - 1));
     const int y = (tile / tx) * TY + (slot >> P.tile.shift);
     Active a;
-    a.sub = threadIdx.x % L;
+    a.sub = lt % L;
     a.py = y;
     a.px = 2 * ax + ((y + color) & 1);
     a.aidx = y * P.wh + ax;
@@ -670,8 +676,9 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
                                           const float* __restrict__ costs, float* __restrict__ ca,
                                           float4* __restrict__ cplane, uint8_t* __restrict__ cflag,
                                           int NA, float (*s_m)[3][kTilePix * L],
-                                          const Frame* s_fr, int ox, int oy, uint32_t& ncc) {
-    const Active a = active_pixel<L>(P, tile, color);
+                                          const Frame* s_fr, int ox, int oy, uint32_t& ncc,
+                                          int lt) {
+    const Active a = active_pixel<L>(P, tile, color, lt);
     if (!a.ok) return;
     const bool lead = a.sub == 0;
     uint32_t flags;
@@ -699,9 +706,9 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
             if (lead && k < 8) cplane[(size_t)k * NA + a.aidx] = pl;
             float m[3];
             plane_m_local(P.rc, pl, (float)ox, (float)oy, m);
-            s_m[k][0][threadIdx.x] = m[0];
-            s_m[k][1][threadIdx.x] = m[1];
-            s_m[k][2][threadIdx.x] = m[2];
+            s_m[k][0][lt] = m[0];
+            s_m[k][1][lt] = m[1];
+            s_m[k][2][lt] = m[2];
         }
     // views outermost: the consecutive NCCs of a view gather from one source
     // image region (the hypotheses are neighbours' planes), which keeps the
@@ -723,7 +730,7 @@ __device__ inline void pm_candidates_body(const PMConst& P, int tile, int color,
                         (cost_default_2 || (k == 0 && v == 0)) ? 2.0f : 0.0f;
                 continue;
             }
-            const float m[3] = {s_m[k][0][threadIdx.x], s_m[k][1][threadIdx.x], s_m[k][2][threadIdx.x]};
+            const float m[3] = {s_m[k][0][lt], s_m[k][1][lt], s_m[k][2][lt]};
             float H[9];
             homography(fr.A, fr.b, m, H);
             const float cv = ncc_eval_mlp<NT, L, PR>(hs, H, src, P.grid());
@@ -1103,19 +1110,29 @@ __device__ inline uint32_t select_pixel(const PMProblem& pr, int color, int iter
     return P.geom ? ((uint32_t)__popc(flags) + 1u) * (uint32_t)__popc(selmask) : 0u;
 }
 
-template <int NT, int L, int PR, bool FUSE>
-__global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(4))) void pm_candidates(const PMProblem* __restrict__ probs,
+// G horizontally adjacent tiles per block (G * 64 * L threads, tile g on
+// threads [g * 64 * L, (g + 1) * 64 * L)): the tiles' gathers from one source
+// region run on one CU and share its L1. Measured on the 10-view 1600x1200
+// ACMP step, FP32 cells: 2 tiles per block 5 % faster than 1 (the gathers
+// run at 0.99 of the measured gather peak instead of 0.94); 2 x 2 tiles as
+// fast as 2 x 1, 4 x 1 and 1 x 2 slower than 2 x 1, 4 x 2 and 2 x 4 (one
+// block per CU) slower than 1. Each tile's arithmetic is the same whatever
+// G is, so results do not depend on it.
+template <int NT, int L, int PR, bool FUSE, int G>
+__global__ __launch_bounds__(G * kTilePix * L) __attribute__((amdgpu_waves_per_eu(4))) void pm_candidates(const PMProblem* __restrict__ probs,
                                                         const WorkTables* __restrict__ tabs,
                                                         int color, int iter,
                                                         uint32_t* __restrict__ queue,
                                                         unsigned long long* work) {
+    static_assert(G == 1 || !FUSE, "the fused selection epilogue runs on one-tile blocks");
     __shared__ int s_item;
-    __shared__ float s_m[9][3][kTilePix * L];
-    __shared__ Frame s_fr[kMaxViews];
-    const WorkTable& t = tabs->t[WT_HALF];
+    __shared__ float s_m[G][9][3][kTilePix * L];
+    __shared__ Frame s_fr[G][kMaxViews];
+    const WorkTable& t = tabs->t[G == 1 ? WT_HALF : WT_HALF_PAIRS];
     const int total = t.start[t.n];
     static_assert(sizeof(s_m) >= sizeof(float) * kMaxViews * kTilePix,
                   "the CDF scratch of the fused selection lives in s_m");
+    const int g = threadIdx.x / (kTilePix * L), lt = threadIdx.x % (kTilePix * L);
     uint32_t ncc = 0, geom_count = 0;
     int q = blockIdx.x % kXcds, tries = 0;
     for (;;) {
@@ -1123,15 +1140,26 @@ __global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(4)
         if (item < 0) break;
         const int p = problem_of(t, item);
         const PMProblem& pr = probs[p];
-        const int tile = item - t.start[p];
-        // the tile spans 2 TX image columns (TX active ones) and TY rows
+        // the item's tiles: tile columns tx0 .. tx0 + G - 1 of tile row ty;
+        // a tile spans 2 TX image columns (TX active ones) and TY rows
         const int TX = pr.P.tile.tx(), TY = pr.P.tile.ty();
         const int ntx = (pr.P.wh + TX - 1) / TX;
-        const int ox = 2 * (tile % ntx) * TX, oy = (tile / ntx) * TY;
-        tile_frames(pr.P, ox, oy, 2 * TX, TY, s_fr);  // next_item's barrier retired the readers
+        const int ngx = (ntx + G - 1) / G;
+        const int grp = item - t.start[p];
+        const int ty = grp / ngx, tx0 = (grp - ty * ngx) * G;
+        // the local frames of the G tiles, one thread per (tile, view)
+        // (next_item's barrier retired the readers)
+        if ((int)threadIdx.x < G * pr.P.nv) {
+            const int gg = (int)threadIdx.x / pr.P.nv, v = (int)threadIdx.x - gg * pr.P.nv;
+            tile_frame(pr.P, v, 2 * (tx0 + gg) * TX, ty * TY, 2 * TX, TY, &s_fr[gg][v]);
+        }
         __syncthreads();
-        pm_candidates_body<NT, L, PR>(pr.P, tile, color, pr.planes, pr.costs, pr.ca, pr.cplane,
-                                      pr.cflag, pr.na, s_m, s_fr, ox, oy, ncc);
+        const int tcx = tx0 + g;
+        const int tile = ty * ntx + tcx;
+        const int ox = 2 * tcx * TX, oy = ty * TY;
+        if (tcx < ntx)
+            pm_candidates_body<NT, L, PR>(pr.P, tile, color, pr.planes, pr.costs, pr.ca, pr.cplane,
+                                          pr.cflag, pr.na, s_m[g], s_fr[g], ox, oy, ncc, lt);
         if constexpr (FUSE) {
             // the selection as the tile's epilogue: the block's candidate
             // table and planes are written (the barrier orders the global
@@ -1144,7 +1172,7 @@ __global__ __launch_bounds__(kTilePix * L) __attribute__((amdgpu_waves_per_eu(4)
             if (threadIdx.x < kTilePix) {
                 const Active a = active_slot(pr.P, tile, color, threadIdx.x);
                 if (a.ok)
-                    geom_count += select_pixel(pr, color, iter, a, &s_m[0][0][0] + threadIdx.x,
+                    geom_count += select_pixel(pr, color, iter, a, &s_m[0][0][0][0] + threadIdx.x,
                                                kTilePix, pr.P.views, true);
             }
         }
@@ -1910,14 +1938,18 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
         GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_problem, dim3(1), dim3(64), 0, st, pr, d_probs + i);
         GMVS_LAUNCH_CHECK();
         nvmax = std::max(nvmax, P.nv);
+        const int htx = (P.wh + P.tile.tx() - 1) / P.tile.tx();
+        const int hty = (h + P.tile.ty() - 1) / P.tile.ty();
         const int counts[WT_NUM] = {P.tile.tiles(w, h), P.tile.tiles(P.wh, h),
-                                    (pr.na + kBlock - 1) / kBlock, (h * w + kBlock - 1) / kBlock};
+                                    (pr.na + kBlock - 1) / kBlock, (h * w + kBlock - 1) / kBlock,
+                                    ((htx + 1) / 2) * hty};
         for (int k = 0; k < WT_NUM; ++k) tabs.t[k].start[i + 1] = tabs.t[k].start[i] + counts[k];
     }
     GMVS_LAUNCH(K_SETUP_VIEW, st, pm_setup_tables, dim3(1), dim3(64), 0, st, tabs, d_tabs);
     GMVS_LAUNCH_CHECK();
 
     const int n_full = tabs.t[WT_FULL].start[n], n_half = tabs.t[WT_HALF].start[n];
+    const int n_pairs = tabs.t[WT_HALF_PAIRS].start[n];
     const int n_act = tabs.t[WT_ACTIVE].start[n], n_pix = tabs.t[WT_PIXELS].start[n];
     const size_t sel_lds = sizeof(ViewDesc) * (size_t)nvmax + sizeof(float) * (size_t)nvmax * kBlock;
     const size_t ref_lds = refine_dyn_bytes(nvmax);
@@ -1925,13 +1957,16 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
     const int li = lanes_of(K_INIT, PR), lc = lanes_of(K_CANDIDATES, PR), lr = lanes_of(K_REFINE, PR);
     auto k_init = li == 2 ? pm_init<kNT, 2, PR> : pm_init<kNT, 4, PR>;
     const bool fuse = g_fuse_select.load() != 0;
-    auto k_cand = fuse ? (lc == 2 ? pm_candidates<kNT, 2, PR, true> : pm_candidates<kNT, 4, PR, true>)
-                       : (lc == 2 ? pm_candidates<kNT, 2, PR, false> : pm_candidates<kNT, 4, PR, false>);
+    // the candidate kernel takes tile pairs unless its selection epilogue is on
+    const int gc = fuse ? 1 : 2;
+    auto k_cand = fuse ? (lc == 2 ? pm_candidates<kNT, 2, PR, true, 1> : pm_candidates<kNT, 4, PR, true, 1>)
+                       : (lc == 2 ? pm_candidates<kNT, 2, PR, false, 2> : pm_candidates<kNT, 4, PR, false, 2>);
     auto k_ref = lr == 2 ? pm_refine<kNT, 2, PR> : pm_refine<kNT, 4, PR>;
-    const int bi = kTilePix * li, bc = kTilePix * lc, br = kTilePix * lr;
+    const int bi = kTilePix * li, bc = gc * kTilePix * lc, br = kTilePix * lr;
     const dim3 g_init(persistent_grid(reinterpret_cast<const void*>(k_init), bi, 0, n_full, K_INIT));
     const dim3 g_cand(
-        persistent_grid(reinterpret_cast<const void*>(k_cand), bc, 0, n_half, K_CANDIDATES));
+        persistent_grid(reinterpret_cast<const void*>(k_cand), bc, 0, gc == 1 ? n_half : n_pairs,
+                        K_CANDIDATES));
     const dim3 g_ref(
         persistent_grid(reinterpret_cast<const void*>(k_ref), br, ref_lds, n_half, K_REFINE));
 

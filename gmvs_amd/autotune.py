@@ -10,8 +10,10 @@ Three knobs change how the work is laid out on the GPU:
   refinement kernels (gmvs_set_lane_width: 2 or 4; 0 = the library's
   default).
 
-The first two never change results; the lane width changes the order of the
-FP32 NCC sums (rounding-level cost differences). The lane width is not
+Blocks per CU never change results. The tile shape and the lane width do at
+rounding level: tap and source positions are formed relative to the tile's
+origin and anchor (the local frame), and the lane width sets the order of the
+FP32 NCC sums. The lane width is not
 searched by default: the short trials start from random planes, whose
 scattered gathers favour 4 lanes, while the converged iterations that
 dominate a full pass favour 2 (the library default, measured on full

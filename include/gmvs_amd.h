@@ -137,8 +137,9 @@ typedef struct gmvs_pm_args {
     float* out_normal;            /* device [H*W*3] or NULL */
     float* out_cost;              /* device [H*W] or NULL */
     int32_t tile_cols;            /* shape of the 64-pixel work tile: columns 4, 8, 16 or 32
-                                     (rows = 64 / cols); 0 = 8 x 8. Results do not depend
-                                     on it (chosen by gmvs_amd.autotune) */
+                                     (rows = 64 / cols); 0 = 8 x 8. Positions are formed
+                                     relative to the tile, so the shape changes costs at
+                                     rounding level (chosen by gmvs_amd.autotune) */
     uint32_t compat;              /* GMVS_COMPAT_* bits; 0 = the reference's behaviour */
 } gmvs_pm_args;
 

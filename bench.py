@@ -294,6 +294,7 @@ def _short_kernel(name: str) -> str:
     name = name.strip().strip('"')
     if name.startswith("void "):
         name = name[5:]
+    name = name.replace("(anonymous namespace)::", "")
     depth = 0
     for i, ch in enumerate(name):  # cut the argument list, keep template args
         if ch == "<":
@@ -316,12 +317,13 @@ def summarize_profile(prof_dir: str) -> str:
 
     out = []
     for path in sorted(glob.glob(os.path.join(prof_dir, "**", "*kernel_stats.csv"), recursive=True)):
-        out.append(f"== {os.path.basename(path)} (top 20 by total time)")
+        out.append(f"== {os.path.basename(path)} (top 20 by total time, then the other gmvs kernels)")
         with open(path, newline="") as f:
             rows = list(csv.DictReader(f))
         rows.sort(key=lambda r: -float(r.get("TotalDurationNs", 0) or 0))
         out.append(f"{'kernel':60s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>10s} {'min_us':>9s} {'max_us':>9s}")
-        for r in rows[:20]:
+        # the top 20, then this library's kernels below them
+        for r in rows[:20] + [r for r in rows[20:] if _short_kernel(r["Name"]).startswith("gmvs::")]:
             out.append(f"{_short_kernel(r['Name'])[:60]:60s} {int(r['Calls']):7d} "
                        f"{float(r['TotalDurationNs']) / 1e6:10.2f} {float(r['AverageNs']) / 1e3:10.1f} "
                        f"{float(r['MinNs']) / 1e3:9.1f} {float(r['MaxNs']) / 1e3:9.1f}")
@@ -562,10 +564,13 @@ def main() -> int:
     ops = ncc_per_launch * OPS_PER_NCC + geom_per_launch * OPS_PER_GEOM
     ach = ops / (avg_ms * 1e-3) / 1e12
     # the tap gathers of that kernel against the measured peak of the same
-    # loads from the vector L1 cache (FP32 cells: 16 B per tap)
+    # loads in the NCC waves' layout (FP32 cells: 16 B per tap); random lane
+    # pairs in an L1-sized window for comparison
     gather_ach = ncc_per_launch * GATHER_BYTES_PER_NCC / (avg_ms * 1e-3) / 1e9
-    gather_peak = N.gather_peak(16)
-    log(f"gather peak (16-byte cells, L1-resident): {gather_peak:.0f} GB/s")
+    gather_peak = N.gather_peak(16, pattern=1)
+    gather_random = N.gather_peak(16, pattern=0)
+    log(f"gather peak (16-byte cells): NCC-wave layout {gather_peak:.0f} GB/s, "
+        f"random lane pairs {gather_random:.0f} GB/s")
     # HBM bytes and VALU issue utilisation per launch of that kernel from the
     # committed rocprofv3 passes over this workload (FETCH_SIZE / WRITE_SIZE
     # scaled by a copy of known size; SQ_INSTS_VALU per launch over one wave
@@ -593,6 +598,7 @@ def main() -> int:
             "ops_per_ncc": OPS_PER_NCC, "hbm": hbm, "valu": valu,
             "gather": {"achieved_GBps": round(gather_ach, 1), "peak_GBps": round(gather_peak, 1),
                        "frac": round(gather_ach / gather_peak, 4),
+                       "random_pairs_GBps": round(gather_random, 1),
                        "bytes_per_ncc": GATHER_BYTES_PER_NCC},
             "kernel": dom_name, "avg_ms": round(avg_ms, 4),
             "ncc_per_launch": int(ncc_per_launch), "geom_per_launch": int(geom_per_launch)}

@@ -230,7 +230,7 @@ def _load() -> ctypes.CDLL:
                                            f32, vp, vp, vp]
     lib.gmvs_gather_peak_buffer_bytes.argtypes = [i32, i32]
     lib.gmvs_gather_peak_buffer_bytes.restype = sz
-    lib.gmvs_gather_peak.argtypes = [i32, i32, i32, vp, sz, ctypes.POINTER(ctypes.c_double), vp]
+    lib.gmvs_gather_peak.argtypes = [i32, i32, i32, i32, vp, sz, ctypes.POINTER(ctypes.c_double), vp]
     lib.gmvs_prior_device_workspace_bytes.argtypes = [i32, i32]
     lib.gmvs_prior_device_workspace_bytes.restype = sz
     lib.gmvs_prior_build_device.argtypes = [ctypes.POINTER(CCamera), vp, vp, vp,
@@ -325,19 +325,22 @@ def set_lane_width(kernel: int, lanes: int) -> None:
     check("gmvs_set_lane_width", lib.gmvs_set_lane_width(kernel, lanes))
 
 
-def gather_peak(cell_bytes: int, window_cells: int = 0, rounds: int = 2000) -> float:
+def gather_peak(cell_bytes: int, pattern: int = 1, window_cells: int = 0,
+                rounds: int = 2000) -> float:
     """Peak gather rate (GB/s) of the NCC tap loads, measured on the current
-    device (gmvs_gather_peak); the window defaults to 2 KB per block."""
+    device (gmvs_gather_peak). pattern 1 (default): the NCC waves' tap-load
+    layout, window 2048 cells per block; pattern 0: random lane pairs, window
+    2 KB per block."""
     import torch
 
-    win = window_cells or 2048 // cell_bytes
+    win = window_cells or (2048 if pattern == 1 else 2048 // cell_bytes)
     nbytes = lib.gmvs_gather_peak_buffer_bytes(cell_bytes, win)
     if nbytes == 0:
         raise ValueError("gather_peak: bad cell size or window")
     buf = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
     out = ctypes.c_double(0.0)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    check("gmvs_gather_peak", lib.gmvs_gather_peak(cell_bytes, win, rounds,
+    check("gmvs_gather_peak", lib.gmvs_gather_peak(cell_bytes, pattern, win, rounds,
                                                      ctypes.c_void_p(buf.data_ptr()), nbytes,
                                                      ctypes.byref(out), stream))
     return out.value

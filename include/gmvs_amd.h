@@ -407,14 +407,18 @@ int gmvs_fusion_planar_mask(const int32_t* seg_ids, const int32_t* planar_mask,
 
 /* No counterpart in the reference: the peak rate of the NCC kernels' tap
  * gathers (structured buffer loads of one 16-byte FP32 cell or one 8-byte
- * 16-bit cell, lane pairs on neighbouring cells, per-block windows of
- * window_cells cells (a power of two) held in the vector L1 cache), in GB/s over the whole
- * device; the reference point of the gather fraction reported by the
- * benchmark. buffer: device, gmvs_gather_peak_buffer_bytes() bytes (its
- * contents are read, never used). Synchronises the stream. */
+ * 16-bit cell) in GB/s over the whole device, within per-block windows of
+ * window_cells cells (a power of two); the reference point of the gather
+ * fraction reported by the benchmark. pattern 0: lane pairs on neighbouring
+ * cells at random places of the window (keep it L1-sized); pattern 1: the
+ * layout of one tap load of an NCC wave (32 pixels of one checkerboard
+ * colour, 8 active columns x 4 rows, lanes of a pixel 2 cells apart) at a
+ * random offset per load, in rows of 64 cells (window_cells >= 2048).
+ * buffer: device, gmvs_gather_peak_buffer_bytes() bytes (its contents are
+ * read, never used). Synchronises the stream. */
 size_t gmvs_gather_peak_buffer_bytes(int32_t cell_bytes, int32_t window_cells);
-int gmvs_gather_peak(int32_t cell_bytes, int32_t window_cells, int32_t rounds, void* buffer,
-                     size_t buffer_bytes, double* out_gbps, void* stream);
+int gmvs_gather_peak(int32_t cell_bytes, int32_t pattern, int32_t window_cells, int32_t rounds,
+                     void* buffer, size_t buffer_bytes, double* out_gbps, void* stream);
 
 /* Kernel timing. No counterpart in the reference (which only prints
  * wall-clock totals, gmvs/scripts/launch.py:125-126,176-177). When enabled,

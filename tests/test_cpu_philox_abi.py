@@ -156,3 +156,16 @@ def test_ctypes_mirrors_match_the_c_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for fname, _ in cls._fields_:
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_gather_peak_rejects_bad_arguments_before_any_device_work():
+    from gmvs_amd import _native as N
+
+    out = ctypes.c_double(0.0)
+    buf = ctypes.c_void_p(1)  # never dereferenced: the checks come first
+    # an unknown pattern; the NCC-wave pattern in a window under 2048 cells
+    assert N.lib.gmvs_gather_peak(16, 2, 4096, 10, buf, 1 << 20, ctypes.byref(out), None) == 1
+    assert b"pattern" in N.lib.gmvs_last_error()
+    assert N.lib.gmvs_gather_peak(16, 1, 1024, 10, buf, 1 << 20, ctypes.byref(out), None) == 1
+    assert b"2048" in N.lib.gmvs_last_error()
+    assert N.lib.gmvs_gather_peak(12, 0, 1024, 10, buf, 1 << 20, ctypes.byref(out), None) == 1

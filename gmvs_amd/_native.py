@@ -314,10 +314,21 @@ def set_fast_taps(enable: bool) -> None:
     check("gmvs_set_fast_taps", lib.gmvs_set_fast_taps(1 if enable else 0))
 
 
+_select_fusion = False
+
+
 def set_select_fusion(enable: bool) -> None:
     """View selection and adoption as the candidate kernel's tile epilogue
     (True) or as their own launch (False, the default); identical results."""
+    global _select_fusion
     check("gmvs_set_select_fusion", lib.gmvs_set_select_fusion(1 if enable else 0))
+    _select_fusion = bool(enable)
+
+
+def select_fusion_enabled() -> bool:
+    """Whether set_select_fusion(True) is in effect (the candidate kernel then
+    holds one tile per block instead of two)."""
+    return _select_fusion
 
 
 def set_lane_width(kernel: int, lanes: int) -> None:

@@ -19,7 +19,8 @@ scattered gathers favour 4 lanes, while the converged iterations that
 dominate a full pass favour 2 (the library default, measured on full
 passes); pass `lanes=(0, 2, 4)` to search it anyway. The blocks-per-CU
 candidates are given as waves per SIMD and converted with the kernel's
-block size (64 pixels x lanes).
+block size (tiles x 64 pixels x lanes; the candidate kernel holds two tiles
+per block).
 
 `autotune(problems)` times short plain passes (random initialisation, one
 iteration) of the given batch on scratch state for each setting and keeps
@@ -96,10 +97,16 @@ def _pick(times: Dict[int, float], default: int, margin: float) -> int:
     return best if times[best] < times[default] * (1.0 - margin) else default
 
 
-def _blocks(waves_per_simd: int, lanes: int) -> int:
+def _tiles_per_block(tune_id: int) -> int:
+    """Tiles per block: the candidate kernel takes pairs of tiles unless the
+    view selection runs as its tile epilogue."""
+    return 2 if tune_id == N.TUNE_CANDIDATES and not N.select_fusion_enabled() else 1
+
+
+def _blocks(waves_per_simd: int, waves_per_block: int) -> int:
     """Blocks per CU for a waves-per-SIMD target (4 SIMDs per CU, blocks of
-    64 pixels x lanes threads = lanes waves); 0 stays 0."""
-    return 0 if waves_per_simd == 0 else max(1, 4 * waves_per_simd // lanes)
+    tiles x 64 pixels x lanes threads = tiles x lanes waves); 0 stays 0."""
+    return 0 if waves_per_simd == 0 else max(1, 4 * waves_per_simd // waves_per_block)
 
 
 def autotune(problems: Sequence[dict], iterations: int = 2, tile_cols: Sequence[int] = TILE_COLS,
@@ -122,7 +129,7 @@ def autotune(problems: Sequence[dict], iterations: int = 2, tile_cols: Sequence[
                                      ("blocks_refine", N.TUNE_REFINE, "pm_refine")):
             times = {}
             for wv in waves:
-                b = _blocks(wv, DEFAULT_LANES[tune_id])
+                b = _blocks(wv, DEFAULT_LANES[tune_id] * _tiles_per_block(tune_id))
                 N.check("gmvs_set_blocks_per_cu", N.lib.gmvs_set_blocks_per_cu(tune_id, b))
                 times[b] = _time(trial, ws, kname)
             choice[name] = _pick(times, 0 if 0 in times else min(times), margin)

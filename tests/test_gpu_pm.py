@@ -77,7 +77,7 @@ def test_cost_batch_matches_fp64(kind):
 
 
 def _pm_gpu(sc, ids, params_kw, init_mode=0, geom=False, depths=None, ref_normal=None,
-            prior=None, state=None, skip_filter=False, precision=0, compat=0):
+            prior=None, state=None, skip_filter=False, precision=0, compat=0, tile_cols=0):
     N, S = _native()
     dev = torch.device("cuda")
     imgs = [sc.images[i].to(dev).contiguous() for i in ids]
@@ -104,7 +104,7 @@ def _pm_gpu(sc, ids, params_kw, init_mode=0, geom=False, depths=None, ref_normal
     S.run_pm(_ccams(sc, ids), imgs, cp, init_mode=init_mode, geom=geom, rng_view=ids[0],
              planes=planes, costs=costs, depths=dd, ref_normal=rn, prior_planes=pp,
              plane_masks=pm, out_depth=od, out_normal=on, out_cost=oc, skip_filter=skip_filter,
-             precision=precision, compat=compat)
+             precision=precision, compat=compat, tile_cols=tile_cols)
     torch.cuda.synchronize()
     return dict(depth=od.cpu().numpy(), normal=on.cpu().numpy(), cost=oc.cpu().numpy(),
                 planes=planes.cpu().numpy())
@@ -144,6 +144,31 @@ def test_pm_plain_hemisphere_matches_oracle():
     rel = np.abs(g["depth"] - o["depth"]) / o["depth"]
     print(f"plain, hemisphere normals: agreement vs FP64 oracle = {agree:.4f}; depth within 0.5 % "
           f"= {np.mean(rel <= 0.005):.4f}")
+    assert agree >= 0.99
+
+
+@pytest.mark.parametrize("lanes_cand,lanes_ref,tile_cols", [(4, 4, 8), (2, 2, 8), (2, 4, 32),
+                                                            (2, 4, 4)])
+def test_pm_plain_launch_shapes_match_oracle(lanes_cand, lanes_ref, tile_cols):
+    """Non-default launch shapes: 4-lane candidate groups (512-thread tile-pair
+    blocks), 2-lane refinement groups, and tile shapes whose tile columns per
+    row are odd (320 x 240 with 32-column tiles: 5 tiles per row, the last
+    pair half empty) or 4 columns wide."""
+    N, _ = _native()
+    sc = T.scene()
+    ids = [0, 1, 2, 3, 4]
+    kw = _params(sc, iters=3)
+    N.set_lane_width(N.TUNE_CANDIDATES, lanes_cand)
+    N.set_lane_width(N.TUNE_REFINE, lanes_ref)
+    try:
+        g = _pm_gpu(sc, ids, kw, tile_cols=tile_cols)
+    finally:
+        N.set_lane_width(N.TUNE_CANDIDATES, 0)
+        N.set_lane_width(N.TUNE_REFINE, 0)
+    o = O.pm_run(T.oracle_cams(sc, ids), [sc.images[i].numpy() for i in ids], O.make_params(**kw),
+                 precision=64, rng_view=0)
+    agree = T.depth_normal_agreement(g["depth"], g["normal"], o["depth"], o["normal"])
+    print(f"lanes {lanes_cand}/{lanes_ref}, tile columns {tile_cols}: agreement vs FP64 = {agree:.4f}")
     assert agree >= 0.99
 
 

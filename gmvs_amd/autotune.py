@@ -14,17 +14,19 @@ Blocks per CU never change results. The tile shape and the lane width do at
 rounding level: tap and source positions are formed relative to the tile's
 origin and anchor (the local frame), and the lane width sets the order of the
 FP32 NCC sums. The lane width is not
-searched by default: the short trials start from random planes, whose
-scattered gathers favour 4 lanes, while the converged iterations that
-dominate a full pass favour 2 (the library default, measured on full
-passes); pass `lanes=(0, 2, 4)` to search it anyway. The blocks-per-CU
+searched by default (the library's defaults, 2 lanes for the candidate
+kernel and 4 for refinement, were measured on full passes); pass
+`lanes=(0, 2, 4)` to search it anyway. The blocks-per-CU
 candidates are given as waves per SIMD and converted with the kernel's
 block size (tiles x 64 pixels x lanes; the candidate kernel holds two tiles
 per block).
 
-`autotune(problems)` times short plain passes (random initialisation, one
-iteration) of the given batch on scratch state for each setting and keeps
-the fastest. Results are cached per batch shape for the process, so a
+`autotune(problems)` runs one short plain pass of the batch from random
+planes (3 iterations), then times, for each setting, plain passes that start
+from its depth and normal maps (INIT_DEPTH_NORMAL, `iterations` iterations)
+on scratch state, and keeps the fastest: iterations from a coherent state
+gather like the converged iterations that dominate a full pass, which
+iterations from random planes do not. Results are cached per batch shape for the process, so a
 pipeline tunes once.
 """
 from __future__ import annotations
@@ -50,18 +52,41 @@ def _key(problems: Sequence[dict]) -> Tuple:
                  for p in problems)
 
 
-def _trial(problems: Sequence[dict], tile_cols: int, iterations: int) -> list:
-    out = []
+def _plain(p: dict, iterations: int, tile_cols: int, **kw) -> dict:
+    """A plain pass (no prior, no geometric term) of problem p on scratch state."""
+    cp = N.CParams.from_buffer_copy(p["params"])
+    cp.max_iterations = iterations
+    cp.planar_prior = False
+    cp.geom_consistency = False
+    d = dict(cams=p["cams"], images=p["images"], params=cp, init_mode=N.INIT_RANDOM, geom=False,
+             rng_view=p.get("rng_view", 0), planes=torch.empty_like(p["planes"]),
+             costs=torch.empty_like(p["costs"]), precision=p.get("precision", N.PREC_FP32),
+             tile_cols=tile_cols)
+    d.update(kw)
+    return d
+
+
+def _warm_state(problems: Sequence[dict], iterations: int = 3) -> list:
+    """(depth, normal) of every problem after a short plain pass from random
+    planes. The trials start from these maps, so they time iterations whose
+    hypotheses gather from coherent footprints, as the converged iterations
+    of a full pass do, not the scattered ones of a random start."""
+    out, batch = [], []
     for p in problems:
-        cp = N.CParams.from_buffer_copy(p["params"])
-        cp.max_iterations = iterations
-        cp.planar_prior = False
-        cp.geom_consistency = False
-        out.append(dict(cams=p["cams"], images=p["images"], params=cp, init_mode=N.INIT_RANDOM,
-                        geom=False, rng_view=p.get("rng_view", 0),
-                        planes=torch.empty_like(p["planes"]), costs=torch.empty_like(p["costs"]),
-                        precision=p.get("precision", N.PREC_FP32), tile_cols=tile_cols))
+        h, w = p["images"][0].shape
+        dev = p["images"][0].device
+        d = torch.empty(h, w, device=dev)
+        n = torch.empty(h, w, 3, device=dev)
+        batch.append(_plain(p, iterations, 0, out_depth=d, out_normal=n))
+        out.append((d, n))
+    S.run_pm_batch(batch, S._Workspace())
     return out
+
+
+def _trial(problems: Sequence[dict], tile_cols: int, iterations: int, state: list) -> list:
+    return [_plain(p, iterations, tile_cols, init_mode=N.INIT_DEPTH_NORMAL,
+                   depths=[d] + [None] * (len(p["images"]) - 1), ref_normal=n)
+            for p, (d, n) in zip(problems, state)]
 
 
 def _time(problems: Sequence[dict], ws: S._Workspace, kernel: Optional[str] = None,
@@ -121,9 +146,10 @@ def autotune(problems: Sequence[dict], iterations: int = 2, tile_cols: Sequence[
     key = _key(problems)
     if key not in _cache:
         ws = S._Workspace()
-        tt = {tc: _time(_trial(problems, tc, iterations), ws) for tc in tile_cols}
+        state = _warm_state(problems)
+        tt = {tc: _time(_trial(problems, tc, iterations, state), ws) for tc in tile_cols}
         best_tc = _pick(tt, 8 if 8 in tt else tile_cols[0], margin)
-        trial = _trial(problems, best_tc, iterations)
+        trial = _trial(problems, best_tc, iterations, state)
         choice = {"tile_cols": best_tc}
         for name, tune_id, kname in (("blocks_candidates", N.TUNE_CANDIDATES, "pm_candidates"),
                                      ("blocks_refine", N.TUNE_REFINE, "pm_refine")):

@@ -166,3 +166,34 @@ def test_pipeline_auto_precision_equals_fp32():
         res.append([t.cpu().numpy() for t in (s1.depth, s1.normal, s2.depth, s2.normal, s2.cost)])
     for a, b in zip(*res):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_autotune_picks_from_its_candidates_and_leaves_state_alone():
+    """The launch-shape tuner on a two-problem batch: the choice comes from
+    the candidate sets, the callers' state tensors are untouched (trials run
+    on scratch state started from a short warm pass), and reset() restores
+    the library defaults, after which a run equals an untuned one."""
+    from gmvs_amd import autotune as AT
+
+    N, S = _native()
+    dev = torch.device("cuda")
+    scenes = [T.scene(), make_scene(200, 136, 5, 4, seed=3407 + 7)]
+    probs, outs = _problems(scenes, [0, 0], 2, 0, dev)
+    before = _snapshot(outs)
+    AT.reset()
+    try:
+        choice = AT.autotune(probs, tile_cols=(8, 16), waves=(0, 2))
+        assert choice["tile_cols"] in (8, 16)
+        assert choice["blocks_candidates"] in (0, AT._blocks(2, 4))
+        assert choice["blocks_refine"] in (0, AT._blocks(2, 4))
+        torch.cuda.synchronize()
+        _assert_same(before, _snapshot(outs))
+    finally:
+        AT.reset()
+    S.run_pm_batch(probs)
+    torch.cuda.synchronize()
+    tuned_off = _snapshot(outs)
+    probs2, outs2 = _problems(scenes, [0, 0], 2, 0, dev)
+    S.run_pm_batch(probs2)
+    torch.cuda.synchronize()
+    _assert_same(tuned_off, _snapshot(outs2))

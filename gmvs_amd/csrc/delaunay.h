@@ -36,6 +36,10 @@
 //      scanned: no point there means a hull edge;
 //   3. repeated from (p, r) until the star closes at q0, else clockwise from
 //      q0 to the other hull edge.
+// Large scans (the half-plane of a hull edge, the circles over empty
+// regions) walk blocks of kBlk x kBlk cells and skip the blocks an
+// occupancy count marks empty; every scan keeps a minimum under a total
+// order, so the cells' visiting order does not change its result.
 // A triangle is written by its lowest-index vertex; triangles are listed by
 // that vertex, then by the index of the next counter-clockwise vertex.
 #pragma once
@@ -83,13 +87,29 @@ __host__ __device__ inline bool in_circle(int64_t ax, int64_t ay, int ia, int64_
 }
 
 // The support points as a cell grid: cell (cx, cy) at cell index
-// cx * ncy + cy holds point pidx[cell] (or -1) at (px[p], py[p]).
+// cx * ncy + cy holds point pidx[cell] (or -1) at (px[p], py[p]). bcnt
+// counts the points of each block of kBlk x kBlk cells (block (bx, by) at
+// bx * nby + by, nby = ceil(ncy / kBlk)); nullptr scans every cell.
+constexpr int kBlk = 8;
+// Ranges of at least this many cells are scanned block by block.
+constexpr int kBlockScanCells = 512;
+
 struct Grid {
     const int* pidx;  // [ncx * ncy]
     const int* px;    // [npts]
     const int* py;
-    int ncx, ncy, cell;
+    const int* bcnt;  // [ceil(ncx / kBlk) * nby] or nullptr
+    int ncx, ncy, cell, nby;
 };
+
+// Points per block of kBlk x kBlk cells, for Grid::bcnt (host).
+inline void block_counts(const int* pidx, int ncx, int ncy, int* bcnt) {
+    const int nbx = (ncx + kBlk - 1) / kBlk, nby = (ncy + kBlk - 1) / kBlk;
+    for (int i = 0; i < nbx * nby; ++i) bcnt[i] = 0;
+    for (int cx = 0; cx < ncx; ++cx)
+        for (int cy = 0; cy < ncy; ++cy)
+            if (pidx[cx * ncy + cy] >= 0) ++bcnt[(cx / kBlk) * nby + cy / kBlk];
+}
 
 // A point's scans are shared by a group of L lanes (L = 1 on the host): the
 // lanes take the cells of a range in turn along its longer side, and the
@@ -121,6 +141,32 @@ __host__ __device__ inline int group_pencil_min(const Grid& g, int p, int q, int
     return r;
 }
 
+// Calls visit(cx, cy) on this lane's share of the cells [x0, x1] x
+// [y0, y1] (the group's lanes cover each cell once). Small ranges: the lanes
+// take the cells in turn along the range's longer side. Large ranges with
+// block counts: block by block, skipping empty blocks, the lanes taking the
+// cells of a block in turn.
+template <int L, class V>
+__host__ __device__ inline void for_cells(const Grid& g, int x0, int x1, int y0, int y1,
+                                          Lanes<L> ln, V&& visit) {
+    if (g.bcnt && (x1 - x0 + 1) * (y1 - y0 + 1) >= kBlockScanCells) {
+        for (int bx = x0 / kBlk; bx <= x1 / kBlk; ++bx)
+            for (int by = y0 / kBlk; by <= y1 / kBlk; ++by) {
+                if (g.bcnt[bx * g.nby + by] == 0) continue;
+                for (int k = ln.lane; k < kBlk * kBlk; k += L) {
+                    const int cx = bx * kBlk + k % kBlk, cy = by * kBlk + k / kBlk;
+                    if (cx >= x0 && cx <= x1 && cy >= y0 && cy <= y1) visit(cx, cy);
+                }
+            }
+    } else if (x1 - x0 > y1 - y0) {
+        for (int cy = y0; cy <= y1; ++cy)
+            for (int cx = x0 + ln.lane; cx <= x1; cx += L) visit(cx, cy);
+    } else {
+        for (int cx = x0; cx <= x1; ++cx)
+            for (int cy = y0 + ln.lane; cy <= y1; cy += L) visit(cx, cy);
+    }
+}
+
 // Grows r over the points of the cells [x0, x1] x [y0, y1]: the point left
 // of p -> q whose circle through p and q is smallest on the left (r = -1:
 // none yet). Every lane of the group returns the same r.
@@ -128,20 +174,13 @@ template <int L, bool REDUCE = true>
 __host__ __device__ inline int left_min(const Grid& g, int p, int q, int x0, int x1, int y0, int y1,
                                         int r, Lanes<L> ln) {
     const int64_t pxv = g.px[p], pyv = g.py[p], qxv = g.px[q], qyv = g.py[q];
-    auto visit = [&](int cx, int cy) {
+    for_cells(g, x0, x1, y0, y1, ln, [&](int cx, int cy) {
         const int s = g.pidx[cx * g.ncy + cy];
         if (s < 0 || s == p || s == q || s == r) return;
         const int64_t sx = g.px[s], sy = g.py[s];
         if (orient(pxv, pyv, qxv, qyv, sx, sy) <= 0) return;
         if (r < 0 || in_circle(pxv, pyv, p, qxv, qyv, q, g.px[r], g.py[r], r, sx, sy, s)) r = s;
-    };
-    if (x1 - x0 > y1 - y0) {
-        for (int cy = y0; cy <= y1; ++cy)
-            for (int cx = x0 + ln.lane; cx <= x1; cx += L) visit(cx, cy);
-    } else {
-        for (int cx = x0; cx <= x1; ++cx)
-            for (int cy = y0 + ln.lane; cy <= y1; cy += L) visit(cx, cy);
-    }
+    });
     if constexpr (L > 1 && REDUCE) r = group_pencil_min(g, p, q, r, ln);
     return r;
 }
@@ -162,19 +201,22 @@ __host__ __device__ inline void cell_span(double lo, double hi, int cell, int n,
     c1 = !(b > 0.0) ? 0 : (b > n - 1 ? n - 1 : (int)b);
 }
 
-// Grows r over every point left of p -> q: the cells of each cell row that
-// can hold a point of the open half-plane (an interval per row from the
-// line's crossing of the row's pixel band, widened by one cell).
+// Grows r over every point left of p -> q: the cells of each band of rows
+// (one cell row, or kBlk rows when the grid has block counts) that can hold
+// a point of the open half-plane (an interval from the line's crossing of
+// the band's pixel rows, widened by one cell).
 template <int L>
 __host__ __device__ inline int left_min_halfplane(const Grid& g, int p, int q, int r, Lanes<L> ln) {
     const double px = g.px[p], py = g.py[p], qx = g.px[q], qy = g.py[q];
     // s is left of p -> q when (qx - px)(sy - py) - (qy - py)(sx - px) > 0
     const double a = -(qy - py);  // coefficient of sx
-    for (int cy = 0; cy < g.ncy; ++cy) {
-        const double ylo = (double)cy * g.cell, yhi = ylo + (g.cell - 1);
+    const int band = g.bcnt ? kBlk : 1;
+    for (int cy0 = 0; cy0 < g.ncy; cy0 += band) {
+        const int cy1 = cy0 + band - 1 < g.ncy ? cy0 + band - 1 : g.ncy - 1;
+        const double ylo = (double)cy0 * g.cell, yhi = (double)cy1 * g.cell + (g.cell - 1);
         int x0 = 0, x1 = g.ncx - 1;
         if (a == 0.0) {
-            // horizontal edge: the whole row or none of it
+            // horizontal edge: the whole band or none of it
             const double f0 = (qx - px) * (ylo - py), f1 = (qx - px) * (yhi - py);
             if (!(f0 > 0.0) && !(f1 > 0.0)) continue;
         } else {
@@ -190,7 +232,7 @@ __host__ __device__ inline int left_min_halfplane(const Grid& g, int p, int q, i
             }
             if (x0 > x1) continue;
         }
-        r = left_min<L, false>(g, p, q, x0, x1, cy, cy, r, ln);  // lane-local
+        r = left_min<L, false>(g, p, q, x0, x1, cy0, cy1, r, ln);  // lane-local
     }
     return group_pencil_min(g, p, q, r, ln);
 }
@@ -263,17 +305,16 @@ __host__ __device__ inline int nearest(const Grid& g, int p, Lanes<L> ln) {
     };
     for (int R = 2;; R *= 4) {
         square(g, pcx, pcy, R, x0, x1, y0, y1);
-        for (int cx = x0; cx <= x1; ++cx)
-            for (int cy = y0 + ln.lane; cy <= y1; cy += L) {
-                const int s = g.pidx[cx * g.ncy + cy];
-                if (s < 0 || s == p) continue;
-                const int64_t dxv = g.px[s] - pxv, dyv = g.py[s] - pyv;
-                const int64_t d2 = dxv * dxv + dyv * dyv;
-                if (better(d2, s, bd, best)) {
-                    best = s;
-                    bd = d2;
-                }
+        for_cells(g, x0, x1, y0, y1, ln, [&](int cx, int cy) {
+            const int s = g.pidx[cx * g.ncy + cy];
+            if (s < 0 || s == p) return;
+            const int64_t dxv = g.px[s] - pxv, dyv = g.py[s] - pyv;
+            const int64_t d2 = dxv * dxv + dyv * dyv;
+            if (better(d2, s, bd, best)) {
+                best = s;
+                bd = d2;
             }
+        });
         if constexpr (L > 1) {
             for (int off = L / 2; off > 0; off >>= 1) {
                 const int ob = __shfl_xor(best, off, L);

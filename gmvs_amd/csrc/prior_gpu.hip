@@ -60,14 +60,15 @@ constexpr int kMaxOwned = 64;
 
 struct PriorLayout {
     size_t cells, tris, keep, rank, tri_of, planes;
-    size_t pidx, px, py, pkeep, prank, cnt, off, tidx, slots, err, total;
-    int ncells, max_tri;
+    size_t pidx, px, py, pkeep, prank, cnt, off, tidx, slots, bcnt, err, total;
+    int ncells, nblocks, max_tri;
 };
 
 PriorLayout prior_layout(int h, int w) {
     PriorLayout L;
     const int ncx = (w + kCell - 1) / kCell, ncy = (h + kCell - 1) / kCell;
     L.ncells = ncx * ncy;
+    L.nblocks = ((ncx + dt::kBlk - 1) / dt::kBlk) * ((ncy + dt::kBlk - 1) / dt::kBlk);
     // a planar triangulation of n points has at most 2n - 5 triangles
     L.max_tri = 2 * L.ncells + 8;
     size_t o = 0;
@@ -102,6 +103,8 @@ PriorLayout prior_layout(int h, int w) {
     o = align_up(o + sizeof(int) * 3 * (size_t)L.max_tri);
     L.slots = o;
     o = align_up(o + sizeof(int2) * kMaxOwned * nc);
+    L.bcnt = o;
+    o = align_up(o + sizeof(int) * (size_t)L.nblocks);
     L.err = o;  // [0] failure flags, [1] triangle count
     o = align_up(o + sizeof(int) * 4);
     L.total = o;
@@ -130,6 +133,21 @@ __global__ void prior_support(const float* __restrict__ cost, int h, int w, int 
             }
         }
     cells[i] = (mc < 0.1f) ? make_int2(bx, by) : make_int2(-1, -1);
+}
+
+// Points per block of dt::kBlk x dt::kBlk cells (dt::Grid::bcnt), one
+// thread per block.
+__global__ void dt_blocks(const int* __restrict__ pidx, int ncx, int ncy, int* __restrict__ bcnt) {
+    const int nby = (ncy + dt::kBlk - 1) / dt::kBlk;
+    const int nb = ((ncx + dt::kBlk - 1) / dt::kBlk) * nby;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int bx = b / nby, by = b - bx * nby;
+    const int x1 = min(ncx, (bx + 1) * dt::kBlk), y1 = min(ncy, (by + 1) * dt::kBlk);
+    int n = 0;
+    for (int cx = bx * dt::kBlk; cx < x1; ++cx)
+        for (int cy = by * dt::kBlk; cy < y1; ++cy) n += pidx[cx * ncy + cy] >= 0;
+    bcnt[b] = n;
 }
 
 // Kept-cell flags of the support cells (cell_xy x < 0: none).
@@ -419,13 +437,19 @@ static int run_delaunay(const PriorLayout& L, char* ws, const int2* cells, int n
     GMVS_LAUNCH_CHECK();
     hipLaunchKernelGGL(dt_points, dim3(nb), dim3(kBlock), 0, st, cells, prank, L.ncells, pidx, px, py);
     GMVS_LAUNCH_CHECK();
+    int* bcnt = reinterpret_cast<int*>(ws + L.bcnt);
+    hipLaunchKernelGGL(dt_blocks, dim3((L.nblocks + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pidx,
+                       ncx, ncy, bcnt);
+    GMVS_LAUNCH_CHECK();
     dt::Grid g;
     g.pidx = pidx;
     g.px = px;
     g.py = py;
+    g.bcnt = bcnt;
     g.ncx = ncx;
     g.ncy = ncy;
     g.cell = kCell;
+    g.nby = (ncy + dt::kBlk - 1) / dt::kBlk;
     const int* npts = prank + L.ncells;
     int2* slots = reinterpret_cast<int2*>(ws + L.slots);
     const int sb = (int)(((size_t)L.ncells * kStarLanes + kBlock - 1) / kBlock);
@@ -596,13 +620,18 @@ extern "C" int gmvs_delaunay_cells_host(const int32_t* cell_xy, int32_t ncx, int
         px.push_back(x);
         py.push_back(y);
     }
+    const int nby = (ncy + dt::kBlk - 1) / dt::kBlk;
+    std::vector<int> bcnt((size_t)((ncx + dt::kBlk - 1) / dt::kBlk) * nby);
+    dt::block_counts(pidx.data(), ncx, ncy, bcnt.data());
     dt::Grid g;
     g.pidx = pidx.data();
     g.px = px.data();
     g.py = py.data();
+    g.bcnt = bcnt.data();
     g.ncx = ncx;
     g.ncy = ncy;
     g.cell = kCell;
+    g.nby = nby;
     std::vector<int> out;
     const int npts = (int)px.size();
     for (int p = 0; p < npts; ++p) {

@@ -131,6 +131,8 @@ def cell_points(ncx, ncy, seed, keep=0.8, mode="random", holes=()):
     (40, 30, 7, 0.8, "random", ((20, 15, 8),)),
     (40, 30, 8, 0.5, "two", ((5, 5, 6), (30, 25, 9))),
     (320, 240, 9, 0.7, "random", ((100, 80, 40), (250, 200, 25), (30, 200, 60))),
+    # grid sides not multiples of the 8-cell occupancy blocks, large scans
+    (83, 61, 10, 0.3, "two", ((40, 30, 20), (0, 60, 15))),
 ])
 def test_star_construction_equals_incremental(ncx, ncy, seed, keep, mode, holes):
     """The device construction's code (each point's star, run on the host)

@@ -1114,7 +1114,9 @@ __device__ inline uint32_t select_pixel(const PMProblem& pr, int color, int iter
 // threads [g * 64 * L, (g + 1) * 64 * L)): the tiles' gathers from one source
 // region run on one CU and share its L1. Measured on the 10-view 1600x1200
 // ACMP step, FP32 cells: 2 tiles per block 5 % faster than 1 (the gathers
-// run at 0.99 of the measured gather peak instead of 0.94); 2 x 2 tiles as
+// run at 1.00 of the measured rate of random lane pairs in an L1-resident
+// window instead of 0.94, a quarter of the L1 accesses instead of 31 %
+// going to the L2); 2 x 2 tiles as
 // fast as 2 x 1, 4 x 1 and 1 x 2 slower than 2 x 1, 4 x 2 and 2 x 4 (one
 // block per CU) slower than 1. Each tile's arithmetic is the same whatever
 // G is, so results do not depend on it.

@@ -314,32 +314,62 @@ __global__ void prior_triangles(const int* __restrict__ tris, const int* __restr
 }
 
 // Exclusive prefix sum of keep[0..n) into rank[0..n], rank[n] = total (also
-// at *total when given); one block, each thread a contiguous chunk. With
-// n_dev, n = min(*n_dev, n_arg) (n_arg bounds the arrays).
-__global__ void prior_rank(const int* __restrict__ keep, int n_arg, const int* __restrict__ n_dev,
-                           int* __restrict__ rank, int* __restrict__ total) {
-    __shared__ int s[kRankThreads];
+// at *total when given); one block. With n_dev, n = min(*n_dev, n_arg)
+// (n_arg bounds the arrays). The block walks the array in chunks of
+// kRankThreads * kRankPer elements, each thread kRankPer consecutive ones
+// (neighbouring threads read neighbouring words): the threads' sums are
+// scanned within each wave by shuffles, the wave totals by the first wave,
+// and a running carry joins the chunks.
+constexpr int kRankPer = 4;
+constexpr int kRankWaves = kRankThreads / 64;
+
+__global__ __launch_bounds__(kRankThreads) void prior_rank(const int* __restrict__ keep, int n_arg,
+                                                           const int* __restrict__ n_dev,
+                                                           int* __restrict__ rank,
+                                                           int* __restrict__ total) {
+    __shared__ int s_wave[kRankWaves];
     const int n = n_dev ? min(*n_dev, n_arg) : n_arg;
-    const int per = (n + kRankThreads - 1) / kRankThreads;
-    const int lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
-    int sum = 0;
-    for (int i = lo; i < hi; ++i) sum += keep[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < kRankThreads; off <<= 1) {  // inclusive Hillis-Steele
-        const int add = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < n; base += kRankThreads * kRankPer) {
+        const int lo = base + (int)threadIdx.x * kRankPer;
+        int v[kRankPer];
+        int sum = 0;
+#pragma unroll
+        for (int j = 0; j < kRankPer; ++j) {
+            v[j] = lo + j < n ? keep[lo + j] : 0;
+            sum += v[j];
+        }
+        int inc = sum;  // inclusive scan over the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(inc, off);
+            if (lane >= off) inc += o;
+        }
+        if (lane == 63) s_wave[wave] = inc;
         __syncthreads();
-        s[threadIdx.x] += add;
+        if (wave == 0) {
+            int w = lane < kRankWaves ? s_wave[lane] : 0;  // inclusive over the waves
+#pragma unroll
+            for (int off = 1; off < kRankWaves; off <<= 1) {
+                const int o = __shfl_up(w, off);
+                if (lane >= off) w += o;
+            }
+            if (lane < kRankWaves) s_wave[lane] = w;
+        }
         __syncthreads();
+        int run = carry + (wave > 0 ? s_wave[wave - 1] : 0) + inc - sum;
+#pragma unroll
+        for (int j = 0; j < kRankPer; ++j) {
+            if (lo + j < n) rank[lo + j] = run;
+            run += v[j];
+        }
+        carry += s_wave[kRankWaves - 1];
+        __syncthreads();  // s_wave is rewritten by the next chunk
     }
-    int run = s[threadIdx.x] - sum;
-    for (int i = lo; i < hi; ++i) {
-        rank[i] = run;
-        run += keep[i];
-    }
-    if (threadIdx.x == kRankThreads - 1) {
-        rank[n] = s[kRankThreads - 1];
-        if (total) *total = s[kRankThreads - 1];
+    if (threadIdx.x == 0) {
+        rank[n] = carry;
+        if (total) *total = carry;
     }
 }
 

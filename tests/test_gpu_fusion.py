@@ -157,3 +157,77 @@ def test_fusion_bit_exact_fusion_configuration():
     assert np.array_equal(masks_g.cpu().numpy().astype(np.uint8), masks_o)
     print(f"fused points (49 views 1600x1200, 20 sources): GPU {total_g}, twin {total_o}")
     assert total_g == total_o > 0
+
+
+def _rig_maps(h, w, f, xs, n, d):
+    """Ground-truth depth maps of the plane n.X + d = 0 seen by cameras with
+    R = I at (x, 0, 0); world normals n (unit, facing the cameras)."""
+    ys, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    rx, ry = (xx - w / 2.0) / f, (ys - h / 2.0) / f
+    ds = []
+    for x in xs:
+        s = -(n[0] * x + d) / (n[0] * rx + n[1] * ry + n[2])
+        ds.append(s.astype(np.float32))
+    nm = np.broadcast_to(np.asarray(n, np.float32), (h, w, 3)).copy()
+    return ds, [nm.copy() for _ in xs]
+
+
+def test_fusion_fp64_check_far_from_the_origin():
+    """The far-depth case of the FP64 check: exact maps of one slanted plane
+    seen by 5 cameras (R = I, 0.25 apart), the whole rig and scene at the
+    world origin and moved by T = (2^20, 2^19, 0) (camera translations
+    -(C + T), exact in FP32; depths and normals unchanged). Every pixel that
+    projects into enough sources is consistent in exact arithmetic. At the
+    origin FP32 and FP64 fuse the same points (to 0.1 %: pixels at the
+    rounding boundaries); far from it the FP64 check still fuses them, while
+    FP32 world points, rounded to 2^-4 units at 2^20, miss the 1-pixel bar
+    (measured: 20,689 / 20,681 at the origin, 6,768 / 20,680 moved, FP32 /
+    FP64). GPU and twin agree bit for bit throughout."""
+    from gmvs_amd import src as S
+
+    h, w, f = 120, 160, 200.0
+    xs = [0.25 * i for i in range(5)]
+    n = np.array([0.2, -0.1, -1.0])
+    n /= np.linalg.norm(n)
+    d = -n[2] * 4.0  # through (0, 0, 4)
+    ds, ns = _rig_maps(h, w, f, xs, n, d)
+    dev = torch.device("cuda")
+    dd = [torch.from_numpy(x).to(dev) for x in ds]
+    nd = [torch.from_numpy(x).to(dev) for x in ns]
+    K = [f, 0, w / 2.0, 0, f, h / 2.0, 0, 0, 1]
+    R = [1.0, 0, 0, 0, 1, 0, 0, 0, 1]
+    counts = {}
+    for shift in (0.0, 1.0):
+        T = np.array([2.0 ** 20, 2.0 ** 19, 0.0]) * shift
+        ts = [-(np.array([x, 0.0, 0.0]) + T) for x in xs]
+        assert all(np.array_equal(t.astype(np.float32).astype(np.float64), t) for t in ts)
+        cams_c = [S.load_camera(K, R, t, h, w, 1.0, 10.0)._c for t in ts]
+        ocams = [O.make_camera(np.array(K).reshape(3, 3), np.eye(3), t, h, w, 1.0, 10.0) for t in ts]
+        for precision in (32, 64):
+            masks_g = torch.zeros(5, h, w, dtype=torch.bool, device=dev)
+            masks_o = np.zeros((5, h, w), np.uint8)
+            total = 0
+            for ref in range(5):
+                ids = [ref] + [i for i in range(5) if i != ref]
+                od = torch.empty(h, w, device=dev)
+                on = torch.empty(h, w, 3, device=dev)
+                S.fuse_view(ids, [cams_c[i] for i in ids], [dd[i] for i in ids],
+                            [nd[i] for i in ids], masks_g, 2, od, on, 1.0, 0.01,
+                            math.radians(30.0), precision=precision)
+                torch.cuda.synchronize()
+                od_o, _, _ = O.fuse_view([ocams[i] for i in ids], [ds[i] for i in ids],
+                                         [ns[i] for i in ids], masks_o, ref, ids[1:], 2, 1.0, 0.01,
+                                         float(np.float32(math.cos(math.radians(30.0)))),
+                                         precision=precision)
+                gd = od.cpu().numpy()
+                assert np.array_equal(gd.view(np.uint32), od_o.view(np.uint32))
+                total += int((gd > 0).sum())
+            counts[(shift, precision)] = total
+    print(f"fused points (shift, precision): {counts}")
+    # a fused pixel masks the source pixels it used, so the total is about
+    # one view's worth of surface points
+    base = counts[(0.0, 64)]
+    assert base > 0.9 * h * w
+    assert abs(counts[(0.0, 32)] - base) <= 0.001 * base
+    assert counts[(1.0, 64)] >= 0.999 * base
+    assert counts[(1.0, 32)] <= 0.5 * base

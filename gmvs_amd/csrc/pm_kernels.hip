@@ -39,7 +39,8 @@ constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robi
 // Lane groups of L lanes evaluate one pixel's NCCs together: the block of an
 // NCC kernel (64 * L threads) holds a tile of 64 pixels, TX x TY with TX = 1 << shift
 // (TX counts active columns in the half-sweep kernels and pixel columns in
-// pm_init). The shape is chosen per problem (gmvs_pm_args.tile_cols).
+// pm_init); pm_candidates' blocks hold two horizontally adjacent tiles. The
+// shape is chosen per problem (gmvs_pm_args.tile_cols).
 struct TileShape {
     int shift;  // log2 of the tile's columns
     __host__ __device__ int tx() const { return 1 << shift; }
@@ -1955,7 +1956,7 @@ int run_batch(const gmvs_pm_args* args, int n, const int* radius, char* ws, cons
     const int n_act = tabs.t[WT_ACTIVE].start[n], n_pix = tabs.t[WT_PIXELS].start[n];
     const size_t sel_lds = sizeof(ViewDesc) * (size_t)nvmax + sizeof(float) * (size_t)nvmax * kBlock;
     const size_t ref_lds = refine_dyn_bytes(nvmax);
-    // lane-group width per kernel: block = 64 pixels x L lanes
+    // lane-group width per kernel: block = tiles x 64 pixels x L lanes
     const int li = lanes_of(K_INIT, PR), lc = lanes_of(K_CANDIDATES, PR), lr = lanes_of(K_REFINE, PR);
     auto k_init = li == 2 ? pm_init<kNT, 2, PR> : pm_init<kNT, 4, PR>;
     const bool fuse = g_fuse_select.load() != 0;
